@@ -1,0 +1,115 @@
+// Internal definitions shared by the HIP translation units of libalto_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "alto_hip.h"
+
+#define ALTO_MAXN 5
+
+namespace alto {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+void set_error_hip(hipError_t e, const char* what);
+
+#define HIP_TRY(expr)                                             \
+    do {                                                          \
+        hipError_t _e = (expr);                                   \
+        if (_e != hipSuccess) {                                   \
+            ::alto::set_error_hip(_e, #expr);                     \
+            return ALTO_ERR_HIP;                                  \
+        }                                                         \
+    } while (0)
+
+#define ALTO_TRY(expr)                       \
+    do {                                     \
+        int _s = (expr);                     \
+        if (_s != ALTO_OK) return _s;        \
+    } while (0)
+
+// ---------------------------------------------------------------- device memory
+// Thin owning wrapper; all device allocations of the library go through it.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int alloc(size_t n);
+    void release();
+    template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------- tensor layout
+// Packed-coordinate layout used by the per-mode streams: mode n occupies bits
+// [off[n], off[n]+bits[n]) of a 64-bit word (pw==1) or of a 128-bit pair (pw==2).
+struct PackLayout {
+    int nmode;
+    int pw;                   // words per packed coordinate (1 or 2)
+    int bits[ALTO_MAXN];
+    int off[ALTO_MAXN];
+};
+
+// Byte lookup table that applies a fixed bit permutation to a W-word key:
+// out = OR_b table[b][byte_b(in)], each entry W words.
+struct BitPermLut {
+    int in_words = 1, out_words = 1;
+    int nb = 0;   // input bytes that carry index bits
+    std::vector<uint64_t> host;   // [in_words*8][256][out_words]
+    DevBuf dev;
+};
+
+struct ModeStream {
+    // Nonzeros of one ALTO chunk sorted by this mode's row (stable), per chunk.
+    DevBuf pk;      // nnz * pw uint64
+    DevBuf val64;   // nnz double (present when FP64 arithmetic was prepared)
+    DevBuf val32;   // nnz float  (present when FP32/BF16 arithmetic was prepared)
+};
+
+}  // namespace alto
+
+struct alto_tensor {
+    int nmode = 0;
+    int W = 1;              // words per ALTO key (LIT width / 64)
+    int nprtn = 1;          // reference partition count
+    int alto_bits = 0;
+    int val_kind = ALTO_VAL_F64;
+    uint64_t nnz = 0;
+    uint64_t dims[ALTO_MAXN] = {0};
+    uint64_t masks[ALTO_MAXN * 2] = {0};
+    uint64_t alto_mask[2] = {0};
+    uint64_t cr_masks[ALTO_MAXN * 2] = {0};
+    uint64_t alto_cr_mask[2] = {0};
+    int is_oidx = 0;
+    std::vector<uint64_t> prtn_ptr;    // nprtn+1 (host copy)
+    std::vector<uint64_t> intervals;   // nprtn*nmode*2 (host copy)
+
+    alto::PackLayout pack;
+    alto::BitPermLut lin;     // packed -> ALTO key
+    alto::BitPermLut delin;   // ALTO key -> packed
+
+    alto::DevBuf key;         // nnz * W uint64, ALTO order, CR bits included
+    alto::DevBuf vals;        // nnz double, ALTO order (ValType cast applied)
+
+    int chunk_log2 = 14;      // nonzeros per chunk of the per-mode streams
+    alto::ModeStream ms[ALTO_MAXN];
+    int device = 0;
+};
+
+namespace alto {
+
+// create.hip
+int build_luts(alto_tensor* t);
+int compute_masks(int nmode, const uint64_t* dims, int W, uint64_t* masks, uint64_t* alto_mask);
+int radix_sort_pairs(uint64_t* keys, int W, uint32_t* payload, uint64_t n, int bits, hipStream_t s);
+int delinearize_all(const alto_tensor* t, uint64_t* coords_dev, hipStream_t s);
+int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s);
+
+}  // namespace alto

@@ -1,0 +1,177 @@
+// Device helpers shared by the stream kernels (MTTKRP, Phi, log-likelihood).
+#pragma once
+
+#include "alto_internal.hpp"
+
+namespace alto {
+
+// bf16 storage: the upper 16 bits of an IEEE binary32
+struct bf16_t {
+    uint16_t x;
+};
+
+template <typename AT> __device__ inline AT to_acc(double v) { return (AT)v; }
+template <typename AT> __device__ inline AT to_acc(float v) { return (AT)v; }
+template <typename AT> __device__ inline AT to_acc(bf16_t v) { return (AT)__uint_as_float((uint32_t)v.x << 16); }
+
+// round-to-nearest-even float -> bf16 (NaN kept quiet)
+__device__ inline bf16_t to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    bf16_t r;
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+        r.x = (uint16_t)((u >> 16) | 0x40u);
+        return r;
+    }
+    u += 0x7fffu + ((u >> 16) & 1u);
+    r.x = (uint16_t)(u >> 16);
+    return r;
+}
+
+template <int PW>
+__device__ inline uint64_t field(const uint64_t (&pk)[PW], int off, int bits)
+{
+    if (bits == 0) return 0;
+    uint64_t msk = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    if (PW == 1) return (pk[0] >> off) & msk;
+    uint64_t v;
+    if (off >= 64) v = pk[PW - 1] >> (off - 64);
+    else if (off == 0) v = pk[0];
+    else v = (pk[0] >> off) | (pk[PW - 1] << (64 - off));
+    return v & msk;
+}
+
+__device__ inline uint64_t shfl_u64(uint64_t x, int src, int width)
+{
+    uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    lo = (uint32_t)__shfl((int)lo, src, width);
+    hi = (uint32_t)__shfl((int)hi, src, width);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline double shfl_val(double x, int src, int width) { return __shfl(x, src, width); }
+__device__ inline float shfl_val(float x, int src, int width) { return __shfl(x, src, width); }
+
+template <int G, typename T>
+__device__ inline T group_sum(T v)
+{
+#pragma unroll
+    for (int o = G / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, G);
+    return v;
+}
+
+__device__ inline void atomic_add_hw(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ inline void atomic_add_hw(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+template <typename AT, typename OT, int E, int G>
+__device__ inline void flush_row(OT* out, uint64_t row, int R, int gl, const AT (&acc)[E])
+{
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int c = gl + q * G;
+        if (c < R) atomic_add_hw(&out[row * (uint64_t)R + c], (OT)acc[q]);
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so that XCD x
+// runs logical blocks [x*nb8, (x+1)*nb8), a contiguous slice of the stream.
+__device__ inline uint64_t xcd_block(uint64_t nblocks_real)
+{
+    const uint64_t nb8 = gridDim.x >> 3;
+    const uint64_t b = blockIdx.x;
+    const uint64_t logical = (b & 7) * nb8 + (b >> 3);
+    return logical < nblocks_real ? logical : ~0ull;
+}
+
+template <typename FT, typename VT, typename OT>
+struct StreamArgs {
+    const uint64_t* pk;
+    const VT* val;
+    const FT* U[ALTO_MAXN];
+    OT* out;
+    uint64_t nnz;
+    uint64_t seg;
+    uint64_t nblocks_real;
+    int nmode, mode, rank;
+    int off[ALTO_MAXN], bits[ALTO_MAXN];
+};
+
+template <typename FT, typename VT, typename OT>
+inline void fill_stream_args(const alto_tensor* t, int mode, int R, int seg, StreamArgs<FT, VT, OT>& a)
+{
+    a.pk = t->ms[mode].pk.as<uint64_t>();
+    a.val = nullptr;
+    a.out = nullptr;
+    for (int m = 0; m < ALTO_MAXN; ++m) {
+        a.U[m] = nullptr;
+        a.off[m] = m < t->nmode ? t->pack.off[m] : 0;
+        a.bits[m] = m < t->nmode ? t->pack.bits[m] : 0;
+    }
+    a.nnz = t->nnz;
+    a.seg = (uint64_t)seg;
+    a.nmode = t->nmode;
+    a.mode = mode;
+    a.rank = R;
+}
+
+inline int group_lanes(int R) { return R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64; }
+inline int default_seg(int R) { (void)R; return 256; }
+
+template <typename A>
+inline dim3 stream_grid(uint64_t nnz, int seg, int G, A& a)
+{
+    uint64_t groups = (nnz + seg - 1) / seg;
+    uint64_t threads = groups * G;
+    uint64_t blocks = (threads + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    a.nblocks_real = blocks;
+    uint64_t nb8 = (blocks + 7) / 8;
+    return dim3((unsigned)(nb8 * 8));
+}
+
+#define RANK_DISPATCH(R_, PW_, ...)                                                      \
+    do {                                                                                 \
+        const int _r = (R_);                                                             \
+        if ((PW_) == 1) {                                                                \
+            constexpr int PW = 1;                                                        \
+            if (_r <= 8) { constexpr int G = 8, E = 1; __VA_ARGS__; }                    \
+            else if (_r <= 16) { constexpr int G = 16, E = 1; __VA_ARGS__; }             \
+            else if (_r <= 32) { constexpr int G = 32, E = 1; __VA_ARGS__; }             \
+            else if (_r <= 64) { constexpr int G = 64, E = 1; __VA_ARGS__; }             \
+            else { constexpr int G = 64, E = 2; __VA_ARGS__; }                           \
+        } else {                                                                         \
+            constexpr int PW = 2;                                                        \
+            if (_r <= 8) { constexpr int G = 8, E = 1; __VA_ARGS__; }                    \
+            else if (_r <= 16) { constexpr int G = 16, E = 1; __VA_ARGS__; }             \
+            else if (_r <= 32) { constexpr int G = 32, E = 1; __VA_ARGS__; }             \
+            else if (_r <= 64) { constexpr int G = 64, E = 1; __VA_ARGS__; }             \
+            else { constexpr int G = 64, E = 2; __VA_ARGS__; }                           \
+        }                                                                                \
+    } while (0)
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ a, uint64_t n, T v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+template <typename T>
+inline int fill_value(T* p, uint64_t n, T v, hipStream_t s)
+{
+    if (n == 0) return ALTO_OK;
+    uint64_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)b), dim3(256), 0, s, p, n, v);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+template <typename T>
+inline int fill_zero(T* p, uint64_t n, hipStream_t s) { return fill_value<T>(p, n, (T)0, s); }
+
+// entry points implemented in mttkrp.hip
+int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* out, int precision, int accumulate,
+                  hipStream_t s, int seg);
+int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s, int seg);
+
+}  // namespace alto

@@ -1,0 +1,162 @@
+/*
+ * libalto_hip — C ABI of the MI355X (gfx950) ALTO engine.
+ *
+ * Drop-in boundary for the reference's header-only templates (IntelLabs/ALTO):
+ * each entry point names the reference interface it replaces.  Plain pointers
+ * and sizes only; every call returns an alto status code (0 = success) and
+ * alto_last_error() describes the last failure of the calling thread.
+ *
+ * Pointer kinds: "host" buffers are ordinary CPU memory (the call
+ * synchronizes before returning, as the reference's synchronous OpenMP calls
+ * do); "device" buffers are HIP device memory of the current device.
+ */
+#ifndef ALTO_HIP_H_
+#define ALTO_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct alto_tensor alto_tensor_t;
+typedef void* alto_stream_t; /* hipStream_t; NULL = the default stream */
+
+enum {
+    ALTO_OK = 0,
+    ALTO_ERR_ARG = -1,
+    ALTO_ERR_HIP = -2,
+    ALTO_ERR_OOM = -3,
+    ALTO_ERR_UNSUPPORTED = -4,
+    ALTO_ERR_BUSY = -5
+};
+
+/* ValType of the ALTO tensor (include/common.hpp:62-69) */
+enum { ALTO_VAL_F64 = 0, ALTO_VAL_U64 = 1 };
+/* arithmetic of MTTKRP / CP iterations */
+enum { ALTO_PREC_F64 = 0, ALTO_PREC_F32 = 1, ALTO_PREC_BF16 = 2 };
+enum { ALTO_PTR_HOST = 0, ALTO_PTR_DEVICE = 1 };
+
+#define ALTO_MAX_MODES 5
+
+const char* alto_last_error(void);
+const char* alto_version(void);
+
+/* ---- ALTO format -------------------------------------------------------- */
+
+/* create_alto<LIT>(SparseTensor*, AltoTensor<LIT>**, int nprtn)
+ * (include/alto.hpp:108-294).  cidx holds nmodes pointers to nnz zero-based
+ * uint64 coordinates (SparseTensor::cidx, include/sptensor.hpp:8-26); vals
+ * holds nnz doubles.  lit_bits selects the ALTO key width (64 or 128, 0 =
+ * smallest that fits).  nprtn is the reference partition count (the OpenMP
+ * thread count at src/main.cpp:353) used for prtn_ptr / intervals / CR bits.
+ * Linearization, sort and partitioning run on the device. */
+int alto_tensor_create(int nmodes, const uint64_t* dims, uint64_t nnz,
+                       const uint64_t* const* cidx, const double* vals, int ptr_kind,
+                       int val_kind, int lit_bits, int nprtn, alto_stream_t stream,
+                       alto_tensor_t** out);
+
+/* destroy_alto<LIT> (include/alto.hpp:296-322) */
+int alto_tensor_destroy(alto_tensor_t* t);
+
+typedef struct {
+    int nmodes, lit_words, alto_bits, nprtn, is_oidx, val_kind;
+    uint64_t nnz;
+    uint64_t dims[ALTO_MAX_MODES];
+    uint64_t mode_masks[ALTO_MAX_MODES * 2]; /* lit_words words per mode */
+    uint64_t alto_mask[2];
+    uint64_t cr_masks[ALTO_MAX_MODES * 2];
+    uint64_t alto_cr_mask[2];
+    uint64_t device_bytes;
+} alto_tensor_info_t;
+
+/* AltoTensor<LIT> scalar fields (include/alto.hpp:27-54) */
+int alto_tensor_info(const alto_tensor_t* t, alto_tensor_info_t* info);
+
+/* Copies AltoTensor<LIT>::idx (nnz*lit_words words, CR bits included),
+ * vals (as double), prtn_ptr (nprtn+1) and prtn_intervals (nprtn*nmodes
+ * {start,stop} pairs) into host buffers; any pointer may be NULL. */
+int alto_tensor_export(const alto_tensor_t* t, uint64_t* idx, double* vals,
+                       uint64_t* prtn_ptr, uint64_t* intervals);
+
+/* Delinearized coordinates (pext of every key per mode, include/alto.hpp:431-455),
+ * nmodes x nnz, into a host buffer. */
+int alto_tensor_delinearize(const alto_tensor_t* t, uint64_t* coords);
+
+/* AltoTensor<LIT>::oidx[mode] (include/alto.hpp:246-285): positions sorted by
+ * the mode's row index; ties in ascending position.  Host buffer of nnz. */
+int alto_tensor_export_oidx(const alto_tensor_t* t, int mode, uint64_t* oidx);
+
+/* Device pdep / pext over arrays (include/bitops.hpp): keys of lit_words
+ * words.  alto_pdep_keys: coords (nmodes x n, device) -> keys (n*lit_words,
+ * device); alto_pext_keys: keys -> coords. */
+int alto_pdep_keys(const alto_tensor_t* t, const uint64_t* coords, uint64_t n, uint64_t* keys,
+                   alto_stream_t stream);
+int alto_pext_keys(const alto_tensor_t* t, const uint64_t* keys, uint64_t n, uint64_t* coords,
+                   alto_stream_t stream);
+
+/* ---- MTTKRP --------------------------------------------------------------- */
+
+/* mttkrp_alto_par(target_mode, factors, rank, at, wlocks, ofibs)
+ * (include/alto.hpp:617-623).  factors: nmodes device pointers to row-major
+ * I_n x rank matrices of the precision's storage type (double / float /
+ * bf16).  out: device I_mode x rank, double for ALTO_PREC_F64, float
+ * otherwise.  accumulate != 0 adds into out as the reference does; 0
+ * overwrites it.  Asynchronous on `stream`. */
+int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* factors, void* out,
+                int precision, int accumulate, alto_stream_t stream);
+
+/* ---- CP-ALS --------------------------------------------------------------- */
+
+typedef struct {
+    double total, mttkrp, pseudoinv, memcopy, normalize, update, fit; /* seconds */
+} alto_cpd_timers_t;
+
+/* cpd_alto<LIT>(AltoTensor*, KruskalModel*, max_iters, epsilon)
+ * (include/cpd.hpp:43-186).  U: nmodes host pointers to row-major
+ * I_n x rank double factors (initial model in, result out); lambda: rank
+ * doubles (out).  fit_trace (may be NULL) receives max_iters doubles.
+ * precision selects factor storage / MTTKRP arithmetic; the R x R solve,
+ * the fit and ||X||^2 stay FP64.  verbose != 0 prints the reference's
+ * per-iteration and timer lines. */
+int alto_cpd_als(alto_tensor_t* t, uint64_t rank, int max_iters, double epsilon, int precision,
+                 double* const* U, double* lambda, double* fit_trace, int* iters,
+                 alto_cpd_timers_t* timers, int verbose, alto_stream_t stream);
+
+/* cpd_fit_alto (include/cpd.hpp:189-271) for a model given in host memory */
+int alto_cpd_fit(alto_tensor_t* t, uint64_t rank, double* const* U, const double* lambda,
+                 double* fit);
+
+/* ---- CP-APR --------------------------------------------------------------- */
+
+typedef struct {
+    double loglik, ls_fit, kkt;
+    int outer_iters, total_inner;
+    int inner_per_mode[ALTO_MAX_MODES];
+    double t_pre, t_apr, t_phi, t_post; /* seconds */
+} alto_apr_result_t;
+
+/* cp_apr_alto<LIT>(AltoTensor*, KruskalModel*, max_iters, max_inner_iters,
+ * epsilon, eps_div_zero, kappa_tol, kappa, dobench) (include/apr.hpp:984-1376).
+ * U / lambda as for alto_cpd_als (FP64). */
+int alto_cp_apr(alto_tensor_t* t, uint64_t rank, int max_iters, int max_inner, double epsilon,
+                double eps_div_zero, double kappa_tol, double kappa, int dobench, double* const* U,
+                double* lambda, alto_apr_result_t* res, int verbose, alto_stream_t stream);
+
+/* CalculatePhi_alto_par (include/apr.hpp:295-303): Phi (device, I_mode x rank,
+ * double) = sum over the mode's nonzeros of (x / max(<B[row,:], pi>, eps)) * pi,
+ * B = factors[mode].  factors: nmodes device pointers (double). */
+int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* factors, double* phi,
+             double eps_div_zero, alto_stream_t stream);
+
+/* ---- device memory helpers (for callers without their own allocator) ---- */
+int alto_device_malloc(void** p, uint64_t bytes);
+int alto_device_free(void* p);
+int alto_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+int alto_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+int alto_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALTO_HIP_H_ */
