@@ -73,6 +73,14 @@ def lib():
         "alto_cpd_als": (i32, [vp, u64, i32, f64, i32, vp, vp, vp, C.POINTER(i32), C.POINTER(CpdTimers), i32, vp]),
         "alto_cpd_fit": (i32, [vp, u64, vp, vp, C.POINTER(f64)]),
         "alto_cp_apr": (i32, [vp, u64, i32, i32, f64, f64, f64, f64, i32, vp, vp, C.POINTER(AprResult), i32, vp]),
+        "alto_update_gram": (i32, [vp, u64, u64, i32, vp, vp]),
+        "alto_cpd_session_create": (i32, [vp, u64, i32, vp, vp, C.POINTER(vp)]),
+        "alto_cpd_session_iterate": (i32, [vp, i32, f64, vp, C.POINTER(i32), C.POINTER(i32), i32]),
+        "alto_cpd_session_get": (i32, [vp, vp, vp, C.POINTER(f64), C.POINTER(CpdTimers)]),
+        "alto_cpd_session_destroy": (i32, [vp]),
+        "alto_pseudo_inverse": (i32, [vp, i32, i32, u64, vp, u64, C.POINTER(i32), vp]),
+        "alto_tune_set": (i32, [i32, i32]),
+        "alto_kruskal_random_init": (i32, [i32, vp, u64, C.c_uint, vp, vp]),
         "alto_device_malloc": (i32, [C.POINTER(vp), u64]),
         "alto_device_free": (i32, [vp]),
         "alto_memcpy_h2d": (i32, [vp, vp, u64]),
@@ -284,17 +292,51 @@ class AltoTensor:
                     t_pre=res.t_pre, t_apr=res.t_apr, t_phi=res.t_phi, t_post=res.t_post)
 
 
+class CpdSession:
+    """Device-resident CP-ALS model over an AltoTensor (alto_cpd_session_*)."""
+
+    def __init__(self, tensor, factors, precision=PREC_F64, stream=None):
+        self.t = tensor
+        self.R = factors[0].shape[1]
+        self.dims = [f.shape[0] for f in factors]
+        fs = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
+        h = C.c_void_p()
+        check(lib().alto_cpd_session_create(tensor.h, self.R, precision, _ptr_array([f.ctypes.data for f in fs]),
+                                            stream, C.byref(h)), "cpd_session_create")
+        self.h = h.value
+
+    def iterate(self, n, epsilon=0.0, verbose=False):
+        fit = np.zeros(max(n, 1), np.float64)
+        ran, conv = C.c_int(0), C.c_int(0)
+        check(lib().alto_cpd_session_iterate(self.h, n, epsilon, fit.ctypes.data, C.byref(ran), C.byref(conv),
+                                             int(verbose)), "cpd_session_iterate")
+        return fit[: ran.value], bool(conv.value)
+
+    def get(self):
+        U = [np.zeros((d, self.R), np.float64) for d in self.dims]
+        lam = np.zeros(self.R, np.float64)
+        fit = C.c_double(0)
+        tm = CpdTimers()
+        check(lib().alto_cpd_session_get(self.h, _ptr_array([u.ctypes.data for u in U]), lam.ctypes.data,
+                                         C.byref(fit), C.byref(tm)), "cpd_session_get")
+        return U, lam, fit.value, {k: getattr(tm, k) for k, _ in CpdTimers._fields_}
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and _lib is not None:
+            _lib.alto_cpd_session_destroy(h)
+            self.h = None
+
+
 def create_alto(dims, cidx, vals, nprtn=8, lit_bits=0, val_kind=VAL_F64):
     return AltoTensor(dims, cidx, vals, nprtn=nprtn, lit_bits=lit_bits, val_kind=val_kind)
 
 
 def kruskal_random_init(dims, rank, seed):
     """KruskalModelRandomInit (src/kruskal_model.cpp:109-125): glibc srand/rand."""
-    libc = C.CDLL("libc.so.6")
-    libc.rand.restype = C.c_int
-    libc.srand(C.c_uint(seed))
-    U = []
-    for d in dims:
-        vals = np.fromiter((libc.rand() for _ in range(int(d) * rank)), dtype=np.float64, count=int(d) * rank)
-        U.append((vals / 2147483647.0).reshape(int(d), rank))
-    return U, np.ones(rank, np.float64)
+    dims = np.ascontiguousarray(dims, dtype=np.uint64)
+    U = [np.zeros((int(d), rank), np.float64) for d in dims]
+    lam = np.zeros(rank, np.float64)
+    check(lib().alto_kruskal_random_init(len(dims), dims.ctypes.data, rank, seed,
+                                         _ptr_array([u.ctypes.data for u in U]), lam.ctypes.data), "random_init")
+    return U, lam

@@ -71,6 +71,7 @@ struct ModeStream {
     DevBuf pk;      // nnz * pw uint64
     DevBuf val64;   // nnz double (present when FP64 arithmetic was prepared)
     DevBuf val32;   // nnz float  (present when FP32/BF16 arithmetic was prepared)
+    int chunk_log2 = -1;  // chunk size the stream was ordered with
 };
 
 }  // namespace alto
@@ -98,7 +99,7 @@ struct alto_tensor {
     alto::DevBuf key;         // nnz * W uint64, ALTO order, CR bits included
     alto::DevBuf vals;        // nnz double, ALTO order (ValType cast applied)
 
-    int chunk_log2 = 14;      // nonzeros per chunk of the per-mode streams
+    int chunk_log2 = 20;      // log2 nonzeros per chunk of the per-mode streams
     alto::ModeStream ms[ALTO_MAXN];
     int device = 0;
 };
@@ -111,5 +112,6 @@ int compute_masks(int nmode, const uint64_t* dims, int W, uint64_t* masks, uint6
 int radix_sort_pairs(uint64_t* keys, int W, uint32_t* payload, uint64_t n, int bits, hipStream_t s);
 int delinearize_all(const alto_tensor* t, uint64_t* coords_dev, hipStream_t s);
 int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s);
+int tuned_chunk_log2();
 
 }  // namespace alto

@@ -1,0 +1,513 @@
+// CP-ALS on the device: the per-iteration loop of cpd_alto
+// (include/cpd.hpp:43-186) with every step on the GPU and only the fit
+// read back per iteration (convergence test and progress line).
+//
+// Per mode j: MTTKRP into M (overwrite) -> [last mode: keep M for the fit]
+// -> V = Hadamard(other Grams), Cholesky (pseudo-inverse fallback) ->
+// U_j = M V^-1 with column statistics -> lambda (2-norm at iteration 0,
+// max-norm after) -> U_j /= lambda (+ BF16 storage copy) -> Gram_j (MFMA).
+// Precisions: FP64 (everything FP64); FP32 (factors, MTTKRP, Grams in FP32,
+// Cholesky / solve arithmetic / fit in FP64); BF16 (FP32 master factors,
+// BF16 copies feed MTTKRP and the Grams, FP32 accumulation).
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <chrono>
+#include <type_traits>
+#include <vector>
+
+#include "alto_internal.hpp"
+#include "dense.hpp"
+#include "kernels_common.hpp"
+
+namespace alto {
+
+struct PhaseTimer {
+    bool on = false;
+    hipStream_t s = nullptr;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> phase;
+    int init(bool enable, hipStream_t st)
+    {
+        on = enable;
+        s = st;
+        return ALTO_OK;
+    }
+    int mark(int ph)
+    {
+        if (!on) return ALTO_OK;
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        HIP_TRY(hipEventRecord(e, s));
+        ev.push_back(e);
+        phase.push_back(ph);
+        return ALTO_OK;
+    }
+    // accumulate the time between consecutive marks into the phase of the later mark
+    int collect(double* acc)
+    {
+        if (!on) return ALTO_OK;
+        for (size_t k = 1; k < ev.size(); ++k) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[k - 1], ev[k]));
+            if (phase[k] >= 0) acc[phase[k]] += ms * 1e-3;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        phase.clear();
+        return ALTO_OK;
+    }
+    ~PhaseTimer()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+enum { PH_MTTKRP = 0, PH_PINV, PH_COPY, PH_NORM, PH_UPDATE, PH_FIT, PH_N };
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename UT, typename ST, typename MT>
+struct CpdState {
+    int N = 0, R = 0, prec = 0;
+    DevBuf U[ALTO_MAXN], S[ALTO_MAXN];
+    DevBuf M, scratch, grams, gpart, L, P, wsA, wsQ, status, psum, pmax, lambda, ipart, fitbuf, nx, nxpart;
+    int nb_solve = 1;
+    UT* u(int m) { return U[m].as<UT>(); }
+    ST* st(int m) { return std::is_same<UT, ST>::value ? reinterpret_cast<ST*>(U[m].p) : S[m].as<ST>(); }
+};
+
+template <typename UT, typename ST, typename MT>
+static int cpd_alloc(alto_tensor* t, int R, CpdState<UT, ST, MT>& c)
+{
+    c.N = t->nmode;
+    c.R = R;
+    uint64_t maxd = 0, maxrows_solve = 0;
+    for (int m = 0; m < t->nmode; ++m) {
+        maxd = std::max<uint64_t>(maxd, t->dims[m]);
+        ALTO_TRY(c.U[m].alloc(t->dims[m] * R * sizeof(UT)));
+        if (!std::is_same<UT, ST>::value) ALTO_TRY(c.S[m].alloc(t->dims[m] * R * sizeof(ST)));
+    }
+    maxrows_solve = maxd;
+    c.nb_solve = solve_blocks(maxrows_solve, R);
+    ALTO_TRY(c.M.alloc(maxd * R * sizeof(MT)));
+    ALTO_TRY(c.scratch.alloc(t->dims[t->nmode - 1] * R * sizeof(MT)));
+    ALTO_TRY(c.grams.alloc((size_t)t->nmode * R * R * 8));
+    ALTO_TRY(c.gpart.alloc((size_t)gram_blocks(maxd) * R * R * 8));
+    ALTO_TRY(c.L.alloc((size_t)R * R * 8));
+    ALTO_TRY(c.P.alloc((size_t)R * R * 8));
+    ALTO_TRY(c.wsA.alloc((size_t)R * R * 8));
+    ALTO_TRY(c.wsQ.alloc((size_t)R * R * 8));
+    ALTO_TRY(c.status.alloc(8));
+    ALTO_TRY(c.psum.alloc((size_t)c.nb_solve * R * 8));
+    ALTO_TRY(c.pmax.alloc((size_t)c.nb_solve * R * 8));
+    ALTO_TRY(c.lambda.alloc((size_t)R * 8));
+    ALTO_TRY(c.ipart.alloc((size_t)64 * R * 8));
+    ALTO_TRY(c.fitbuf.alloc(4 * 8));
+    ALTO_TRY(c.nx.alloc(8));
+    ALTO_TRY(c.nxpart.alloc(256 * 8));
+    return ALTO_OK;
+}
+
+static bool gram_f64(int prec) { return prec == ALTO_PREC_F64; }
+
+template <typename UT, typename ST, typename MT>
+static int cpd_upload(alto_tensor* t, CpdState<UT, ST, MT>& c, double* const* Uh, hipStream_t s)
+{
+    const int R = c.R;
+    std::vector<UT> tmp;
+    DevBuf ones;
+    ALTO_TRY(ones.alloc(R * 8));
+    ALTO_TRY(fill_value<double>(ones.as<double>(), R, 1.0, s));
+    for (int m = 0; m < c.N; ++m) {
+        uint64_t n = t->dims[m] * R;
+        if (std::is_same<UT, double>::value) {
+            HIP_TRY(hipMemcpyAsync(c.U[m].p, Uh[m], n * 8, hipMemcpyHostToDevice, s));
+        } else {
+            tmp.resize(n);
+            for (uint64_t k = 0; k < n; ++k) tmp[k] = (UT)Uh[m][k];
+            HIP_TRY(hipMemcpyAsync(c.U[m].p, tmp.data(), n * sizeof(UT), hipMemcpyHostToDevice, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        if (!std::is_same<UT, ST>::value)
+            ALTO_TRY((scale_device<UT, ST>(c.u(m), t->dims[m], R, ones.as<double>(), c.st(m), s)));
+        ALTO_TRY(gram_device<ST>(c.st(m), t->dims[m], R, c.grams.template as<double>() + (size_t)m * R * R,
+                                 c.gpart.template as<double>(), s, gram_f64(c.prec)));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return ALTO_OK;
+}
+
+template <typename UT, typename ST, typename MT>
+static int cpd_download(alto_tensor* t, CpdState<UT, ST, MT>& c, double* const* Uh, double* lambda_h, hipStream_t s)
+{
+    const int R = c.R;
+    std::vector<UT> tmp;
+    for (int m = 0; m < c.N; ++m) {
+        uint64_t n = t->dims[m] * R;
+        if (std::is_same<UT, double>::value) {
+            HIP_TRY(hipMemcpyAsync(Uh[m], c.U[m].p, n * 8, hipMemcpyDeviceToHost, s));
+        } else {
+            tmp.resize(n);
+            HIP_TRY(hipMemcpyAsync(tmp.data(), c.U[m].p, n * sizeof(UT), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            for (uint64_t k = 0; k < n; ++k) Uh[m][k] = (double)tmp[k];
+        }
+    }
+    if (lambda_h) HIP_TRY(hipMemcpyAsync(lambda_h, c.lambda.p, R * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ALTO_OK;
+}
+
+// one mode update (cpd_alto's loop body, include/cpd.hpp:103-138)
+template <typename UT, typename ST, typename MT>
+static int cpd_mode_step(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it, PhaseTimer& pt, hipStream_t s)
+{
+    const int R = c.R, N = c.N;
+    const void* fptr[ALTO_MAXN];
+    for (int m = 0; m < N; ++m) fptr[m] = c.st(m);
+    ALTO_TRY(pt.mark(-1));
+    ALTO_TRY(mttkrp_device(t, j, R, fptr, c.M.p, c.prec, 0, s, 0));
+    ALTO_TRY(pt.mark(PH_MTTKRP));
+    if (j == N - 1)
+        HIP_TRY(hipMemcpyAsync(c.scratch.p, c.M.p, t->dims[j] * R * sizeof(MT), hipMemcpyDeviceToDevice, s));
+    ALTO_TRY(pt.mark(PH_COPY));
+    ALTO_TRY(chol_device(c.grams.template as<double>(), N, j, R, c.L.template as<double>(), c.P.template as<double>(),
+                         c.wsA.template as<double>(), c.wsQ.template as<double>(), c.status.template as<int>(), s));
+    ALTO_TRY((solve_device<MT, UT>(c.M.template as<MT>(), t->dims[j], R, c.L.template as<double>(),
+                                   c.P.template as<double>(), c.status.template as<int>(), c.u(j),
+                                   c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, s)));
+    ALTO_TRY(pt.mark(PH_PINV));
+    ALTO_TRY(lambda_device(c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, R, it == 0 ? 0 : 1,
+                           c.lambda.template as<double>(), s));
+    ALTO_TRY((scale_device<UT, ST>(c.u(j), t->dims[j], R, c.lambda.template as<double>(),
+                                   std::is_same<UT, ST>::value ? nullptr : c.st(j), s)));
+    ALTO_TRY(pt.mark(PH_NORM));
+    ALTO_TRY(gram_device<ST>(c.st(j), t->dims[j], R, c.grams.template as<double>() + (size_t)j * R * R,
+                             c.gpart.template as<double>(), s, gram_f64(c.prec)));
+    ALTO_TRY(pt.mark(PH_UPDATE));
+    return ALTO_OK;
+}
+
+template <typename UT, typename ST, typename MT>
+static int cpd_fit_step(alto_tensor* t, CpdState<UT, ST, MT>& c, double normX, hipStream_t s)
+{
+    const int N = c.N, R = c.R;
+    ALTO_TRY((inner_partial_device<UT, MT>(c.u(N - 1), c.scratch.template as<MT>(), t->dims[N - 1], R,
+                                           c.ipart.template as<double>(), s)));
+    ALTO_TRY(fit_device(c.grams.template as<double>(), N, R, c.lambda.template as<double>(), c.ipart.template as<double>(),
+                        normX, c.fitbuf.template as<double>(), s));
+    return ALTO_OK;
+}
+
+// A CP-ALS model resident on the device (factors, Grams, workspaces) that
+// can be iterated in several calls; the iteration counter and the previous
+// fit persist, so iterate(a) then iterate(b) equals iterate(a + b).
+struct CpdSessionBase {
+    alto_tensor* t = nullptr;
+    int R = 0, prec = 0, it = 0;
+    double prev_fit = 0, fit = 0, normX = 0;
+    bool converged = false;
+    hipStream_t s = nullptr;
+    double acc[PH_N] = {0};
+    double total = 0;
+    virtual ~CpdSessionBase() {}
+    virtual int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) = 0;
+    virtual int get(double* const* Uh, double* lambda_h) = 0;
+};
+
+template <typename UT, typename ST, typename MT>
+struct CpdSession : CpdSessionBase {
+    CpdState<UT, ST, MT> c;
+    int init(alto_tensor* tt, int rank, int precision, double* const* Uh, hipStream_t st)
+    {
+        t = tt;
+        R = rank;
+        prec = precision;
+        s = st;
+        c.prec = prec;
+        ALTO_TRY(prepare_mode_streams(t, prec == ALTO_PREC_F64, prec != ALTO_PREC_F64, s));
+        ALTO_TRY(cpd_alloc(t, R, c));
+        ALTO_TRY(cpd_upload(t, c, Uh, s));
+        ALTO_TRY(fill_value<double>(c.lambda.template as<double>(), R, 1.0, s));
+        ALTO_TRY(normsq_device(t->vals.as<double>(), t->nnz, c.nxpart.template as<double>(), c.nx.template as<double>(), s));
+        HIP_TRY(hipMemcpyAsync(&normX, c.nx.p, 8, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return ALTO_OK;
+    }
+    int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) override
+    {
+        PhaseTimer pt;
+        ALTO_TRY(pt.init(true, s));
+        int done = 0;
+        const double t0 = now_s();
+        for (int k = 0; k < max_iters && !converged; ++k, ++it) {
+            const double ti = now_s();
+            const double mt_before = acc[PH_MTTKRP];
+            for (int j = 0; j < c.N; ++j) ALTO_TRY(cpd_mode_step(t, c, j, it, pt, s));
+            ALTO_TRY(pt.mark(-1));
+            ALTO_TRY(cpd_fit_step(t, c, normX, s));
+            ALTO_TRY(pt.mark(PH_FIT));
+            double fb[3];
+            HIP_TRY(hipMemcpyAsync(fb, c.fitbuf.p, 24, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            ALTO_TRY(pt.collect(acc));
+            fit = fb[0];
+            if (fit_trace) fit_trace[k] = fit;
+            ++done;
+            if (verbose)
+                printf("it: %d\t fit: %g\t fit-delta: %g\ttime(for MTTKRP): %.4f s (%.4f s)\n", it, fit,
+                       fabs(prev_fit - fit), now_s() - ti, acc[PH_MTTKRP] - mt_before);
+            if (it > 0 && fabs(prev_fit - fit) < eps) converged = true;
+            prev_fit = fit;
+        }
+        total += now_s() - t0;
+        if (ran) *ran = done;
+        return ALTO_OK;
+    }
+    int get(double* const* Uh, double* lambda_h) override
+    {
+        if (Uh) return cpd_download(t, c, Uh, lambda_h, s);
+        if (lambda_h) {
+            HIP_TRY(hipMemcpyAsync(lambda_h, c.lambda.p, R * 8, hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+        }
+        return ALTO_OK;
+    }
+};
+
+static int session_new(alto_tensor* t, int R, int prec, double* const* Uh, hipStream_t s, CpdSessionBase** out)
+{
+    if (R < 1 || R > 128) {
+        set_error("cpd: rank must be 1..128");
+        return ALTO_ERR_ARG;
+    }
+    CpdSessionBase* b = nullptr;
+    int st = ALTO_OK;
+    switch (prec) {
+    case ALTO_PREC_F64: {
+        auto* x = new CpdSession<double, double, double>();
+        st = x->init(t, R, prec, Uh, s);
+        b = x;
+        break;
+    }
+    case ALTO_PREC_F32: {
+        auto* x = new CpdSession<float, float, float>();
+        st = x->init(t, R, prec, Uh, s);
+        b = x;
+        break;
+    }
+    case ALTO_PREC_BF16: {
+        auto* x = new CpdSession<float, bf16_t, float>();
+        st = x->init(t, R, prec, Uh, s);
+        b = x;
+        break;
+    }
+    default:
+        set_error("cpd: unknown precision %d", prec);
+        return ALTO_ERR_ARG;
+    }
+    if (st != ALTO_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return ALTO_OK;
+}
+
+static void print_totals(const CpdSessionBase* b, int iters)
+{
+    const double* acc = b->acc;
+    printf("Total time (for MTTKRP):\t %.4f s (%.4f s)\n", b->total, acc[PH_MTTKRP]);
+    printf("Total     MTTKRP    PseudoInv MemCopy   Normalize Update    Fit\n");
+    printf("%07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f\n", b->total, acc[PH_MTTKRP], acc[PH_PINV],
+           acc[PH_COPY], acc[PH_NORM], acc[PH_UPDATE], acc[PH_FIT]);
+    if (iters < 1) iters = 1;
+    printf("Per iteration\n%07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f   %07.4f\n", b->total / iters,
+           acc[PH_MTTKRP] / iters, acc[PH_PINV] / iters, acc[PH_COPY] / iters, acc[PH_NORM] / iters,
+           acc[PH_UPDATE] / iters, acc[PH_FIT] / iters);
+}
+
+static void fill_timers(const CpdSessionBase* b, alto_cpd_timers_t* tm)
+{
+    tm->total = b->total;
+    tm->mttkrp = b->acc[PH_MTTKRP];
+    tm->pseudoinv = b->acc[PH_PINV];
+    tm->memcopy = b->acc[PH_COPY];
+    tm->normalize = b->acc[PH_NORM];
+    tm->update = b->acc[PH_UPDATE];
+    tm->fit = b->acc[PH_FIT];
+}
+
+int cpd_als_impl(alto_tensor* t, int R, int max_iters, double eps, int prec, double* const* Uh, double* lambda_h,
+                 double* fit_trace, int* iters, alto_cpd_timers_t* tm, int verbose, hipStream_t s)
+{
+    if (max_iters < 1) {
+        set_error("cpd_als: max_iters must be >= 1");
+        return ALTO_ERR_ARG;
+    }
+    CpdSessionBase* b = nullptr;
+    ALTO_TRY(session_new(t, R, prec, Uh, s, &b));
+    std::unique_ptr<CpdSessionBase> guard(b);
+    if (verbose) printf("Running ALTO CP-ALS with %d max iterations and %.2e epsilon\n", max_iters, eps);
+    int ran = 0;
+    ALTO_TRY(b->iterate(max_iters, eps, fit_trace, &ran, verbose));
+    ALTO_TRY(b->get(Uh, lambda_h));
+    if (iters) *iters = ran;
+    if (tm) fill_timers(b, tm);
+    if (verbose) print_totals(b, ran);
+    return ALTO_OK;
+}
+
+// cpd_fit_alto for a model in host memory: Grams, last-mode MTTKRP, fit
+int cpd_fit_impl(alto_tensor* t, int R, double* const* Uh, const double* lambda_h, double* fit)
+{
+    hipStream_t s = nullptr;
+    CpdState<double, double, double> c;
+    c.prec = ALTO_PREC_F64;
+    ALTO_TRY(prepare_mode_streams(t, 1, 0, s));
+    ALTO_TRY(cpd_alloc(t, R, c));
+    ALTO_TRY(cpd_upload(t, c, Uh, s));
+    HIP_TRY(hipMemcpy(c.lambda.p, lambda_h, R * 8, hipMemcpyHostToDevice));
+    const void* fptr[ALTO_MAXN];
+    for (int m = 0; m < c.N; ++m) fptr[m] = c.st(m);
+    ALTO_TRY(mttkrp_device(t, c.N - 1, R, fptr, c.scratch.p, ALTO_PREC_F64, 0, s, 0));
+    ALTO_TRY(normsq_device(t->vals.as<double>(), t->nnz, c.nxpart.template as<double>(), c.nx.template as<double>(), s));
+    double normX = 0;
+    HIP_TRY(hipMemcpy(&normX, c.nx.p, 8, hipMemcpyDeviceToHost));
+    ALTO_TRY(cpd_fit_step(t, c, normX, s));
+    double fb[3];
+    HIP_TRY(hipMemcpy(fb, c.fitbuf.p, 24, hipMemcpyDeviceToHost));
+    *fit = fb[0];
+    return ALTO_OK;
+}
+
+}  // namespace alto
+
+using namespace alto;
+
+extern "C" {
+
+int alto_cpd_als(alto_tensor_t* t, uint64_t rank, int max_iters, double epsilon, int precision, double* const* U,
+                 double* lambda, double* fit_trace, int* iters, alto_cpd_timers_t* timers, int verbose,
+                 alto_stream_t stream)
+{
+    if (!t || !U || !lambda) {
+        set_error("alto_cpd_als: null argument");
+        return ALTO_ERR_ARG;
+    }
+    return cpd_als_impl(t, (int)rank, max_iters, epsilon, precision, U, lambda, fit_trace, iters, timers, verbose,
+                        (hipStream_t)stream);
+}
+
+int alto_cpd_session_create(alto_tensor_t* t, uint64_t rank, int precision, double* const* U, alto_stream_t stream,
+                            alto_cpd_session_t** out)
+{
+    if (!t || !U || !out) {
+        set_error("alto_cpd_session_create: null argument");
+        return ALTO_ERR_ARG;
+    }
+    CpdSessionBase* b = nullptr;
+    ALTO_TRY(session_new(t, (int)rank, precision, U, (hipStream_t)stream, &b));
+    *out = reinterpret_cast<alto_cpd_session_t*>(b);
+    return ALTO_OK;
+}
+
+int alto_cpd_session_iterate(alto_cpd_session_t* sess, int max_iters, double epsilon, double* fit_trace, int* iters,
+                             int* converged, int verbose)
+{
+    if (!sess || max_iters < 0) {
+        set_error("alto_cpd_session_iterate: invalid argument");
+        return ALTO_ERR_ARG;
+    }
+    CpdSessionBase* b = reinterpret_cast<CpdSessionBase*>(sess);
+    ALTO_TRY(b->iterate(max_iters, epsilon, fit_trace, iters, verbose));
+    if (converged) *converged = b->converged ? 1 : 0;
+    return ALTO_OK;
+}
+
+int alto_cpd_session_get(alto_cpd_session_t* sess, double* const* U, double* lambda, double* fit,
+                         alto_cpd_timers_t* timers)
+{
+    if (!sess) {
+        set_error("alto_cpd_session_get: null session");
+        return ALTO_ERR_ARG;
+    }
+    CpdSessionBase* b = reinterpret_cast<CpdSessionBase*>(sess);
+    ALTO_TRY(b->get(U, lambda));
+    if (fit) *fit = b->fit;
+    if (timers) fill_timers(b, timers);
+    return ALTO_OK;
+}
+
+int alto_cpd_session_destroy(alto_cpd_session_t* sess)
+{
+    delete reinterpret_cast<CpdSessionBase*>(sess);
+    return ALTO_OK;
+}
+
+int alto_cpd_fit(alto_tensor_t* t, uint64_t rank, double* const* U, const double* lambda, double* fit)
+{
+    if (!t || !U || !lambda || !fit) {
+        set_error("alto_cpd_fit: null argument");
+        return ALTO_ERR_ARG;
+    }
+    return cpd_fit_impl(t, (int)rank, U, lambda, fit);
+}
+
+int alto_pseudo_inverse(const double* grams, int nmodes, int mode, uint64_t rank, double* U, uint64_t dim,
+                        int* used_fallback, alto_stream_t stream)
+{
+    if (!grams || !U || rank < 1 || rank > 128 || mode < 0 || mode >= nmodes || nmodes > ALTO_MAXN) {
+        set_error("alto_pseudo_inverse: invalid argument");
+        return ALTO_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int R = (int)rank;
+    DevBuf L, P, A, Q, st, psum, pmax;
+    const int nb = solve_blocks(dim, R);
+    ALTO_TRY(L.alloc((size_t)R * R * 8));
+    ALTO_TRY(P.alloc((size_t)R * R * 8));
+    ALTO_TRY(A.alloc((size_t)R * R * 8));
+    ALTO_TRY(Q.alloc((size_t)R * R * 8));
+    ALTO_TRY(st.alloc(8));
+    ALTO_TRY(psum.alloc((size_t)nb * R * 8));
+    ALTO_TRY(pmax.alloc((size_t)nb * R * 8));
+    ALTO_TRY(chol_device(grams, nmodes, mode, R, L.as<double>(), P.as<double>(), A.as<double>(), Q.as<double>(),
+                         st.as<int>(), s));
+    ALTO_TRY((solve_device<double, double>(U, dim, R, L.as<double>(), P.as<double>(), st.as<int>(), U,
+                                           psum.as<double>(), pmax.as<double>(), nb, s)));
+    int h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, st.p, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (used_fallback) *used_fallback = h;
+    return ALTO_OK;
+}
+
+int alto_update_gram(const void* U, uint64_t dim, uint64_t rank, int precision, double* gram, alto_stream_t stream)
+{
+    if (!U || !gram || rank < 1 || rank > 128) {
+        set_error("alto_update_gram: invalid argument");
+        return ALTO_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf part;
+    ALTO_TRY(part.alloc((size_t)gram_blocks(dim) * rank * rank * 8));
+    int st;
+    if (precision == ALTO_PREC_F64)
+        st = gram_device<double>((const double*)U, dim, (int)rank, gram, part.as<double>(), s, true);
+    else if (precision == ALTO_PREC_F32)
+        st = gram_device<float>((const float*)U, dim, (int)rank, gram, part.as<double>(), s, false);
+    else
+        st = gram_device<bf16_t>((const bf16_t*)U, dim, (int)rank, gram, part.as<double>(), s, false);
+    if (st != ALTO_OK) return st;
+    HIP_TRY(hipStreamSynchronize(s));
+    return ALTO_OK;
+}
+
+}  // extern "C"

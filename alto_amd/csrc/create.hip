@@ -671,9 +671,18 @@ int delinearize_all(const alto_tensor* t, uint64_t* coords, hipStream_t s)
 int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
 {
     uint64_t n = t->nnz;
+    const int want_chunk = tuned_chunk_log2();
+    if (want_chunk > 0 && want_chunk != t->chunk_log2) t->chunk_log2 = want_chunk;
     bool any = false;
-    for (int m = 0; m < t->nmode; ++m)
-        if ((need64 && !t->ms[m].val64.p) || (need32 && !t->ms[m].val32.p) || !t->ms[m].pk.p) any = true;
+    for (int m = 0; m < t->nmode; ++m) {
+        ModeStream& M = t->ms[m];
+        if (M.chunk_log2 != t->chunk_log2) {  // reorder for a new chunk size
+            M.pk.release();
+            M.val64.release();
+            M.val32.release();
+        }
+        if ((need64 && !M.val64.p) || (need32 && !M.val32.p) || !M.pk.p) any = true;
+    }
     if (!any) return ALTO_OK;
     DevBuf ck, pay;
     ALTO_TRY(ck.alloc(n * 8));
@@ -683,8 +692,8 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     while ((1ull << cbits) < nchunks) ++cbits;
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
-        bool fresh = !M.pk.p;
-        if (fresh) ALTO_TRY(M.pk.alloc(n * t->pack.pw * 8));
+        M.chunk_log2 = t->chunk_log2;
+        if (!M.pk.p) ALTO_TRY(M.pk.alloc(n * t->pack.pw * 8));
         if (need64 && !M.val64.p) ALTO_TRY(M.val64.alloc(n * 8));
         if (need32 && !M.val32.p) ALTO_TRY(M.val32.alloc(n * 4));
         DISPATCH_WPW(t->W, t->pack.pw, {

@@ -1,0 +1,567 @@
+// Dense R x R and I x R kernels of CP-ALS / CP-APR on gfx950.
+//
+//  - Gram matrices U^T U on the matrix cores: v_mfma_f64_16x16x4f64 for FP64
+//    factors, v_mfma_f32_16x16x4f32 for FP32 / BF16 factors; per-block
+//    partial Grams reduced in a fixed order (deterministic).
+//    Reference: update_gram = SYRK (src/kruskal_model.cpp:11-24).
+//  - V = Hadamard product of the other modes' Grams and its Cholesky factor
+//    in one workgroup (FP64); when Cholesky fails, the minimum-norm
+//    pseudo-inverse from a Jacobi eigen-decomposition.
+//    Reference: pseudo_inverse (include/cpd.hpp:546-646).
+//  - Row solves U_n[i,:] = M[i,:] V^-1 (two triangular solves per row, one
+//    lane group per row), fused with the per-block column norms.
+//  - Column normalisation (2-norm / max-norm), the fit, ||X||^2.
+//    Reference: KruskalModelNorm (src/kruskal_model.cpp:224-341),
+//    cpd_fit_alto (include/cpd.hpp:189-271).
+#include <math.h>
+
+#include <type_traits>
+
+#include "alto_internal.hpp"
+#include "dense.hpp"
+#include "kernels_common.hpp"
+
+namespace alto {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> __device__ inline double as_f64(T v) { return (double)to_acc<double>(v); }
+
+// ------------------------------------------------------------------ Gram
+// Each wave owns 16x16 output tiles (t = wave, wave+4, ...).  One MFMA step
+// consumes 4 rows: lane l supplies U[i0 + l/16][a0 + l%16] as the A operand
+// (A = U^T block, 16 x 4) and U[i0 + l/16][b0 + l%16] as the B operand.
+// The 16x16 result is spread over the 4 result registers of each lane (see
+// the store below for the per-instruction row mapping).
+constexpr int GRAM_ROWS = 256;  // rows per block per sweep
+constexpr int GRAM_MAXT = 16;   // tiles per wave (R <= 128)
+
+template <typename T, bool F64>
+__global__ void __launch_bounds__(256) k_gram_partial(const T* __restrict__ U, uint64_t I, int R, double* __restrict__ part)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int T16 = (R + 15) / 16;
+    const int ntiles = T16 * T16;
+    const int r16 = lane & 15, kq = lane >> 4;
+    using acc_t = typename std::conditional<F64, d4_t, f4_t>::type;
+    acc_t acc[GRAM_MAXT];
+#pragma unroll
+    for (int t = 0; t < GRAM_MAXT; ++t) acc[t] = acc_t{0, 0, 0, 0};
+    for (uint64_t base = (uint64_t)blockIdx.x * GRAM_ROWS; base < I; base += (uint64_t)gridDim.x * GRAM_ROWS) {
+        const uint64_t end = base + GRAM_ROWS < I ? base + GRAM_ROWS : I;
+        for (uint64_t i0 = base; i0 < end; i0 += 4) {
+            const uint64_t i = i0 + kq;
+            const bool rv = i < end;
+#pragma unroll
+            for (int tt = 0; tt < GRAM_MAXT; ++tt) {
+                const int t = wave + 4 * tt;
+                if (t < ntiles) {
+                    const int a = (t / T16) * 16 + r16, b = (t % T16) * 16 + r16;
+                    const double av = rv && a < R ? as_f64(U[i * R + a]) : 0.0;
+                    const double bv = rv && b < R ? as_f64(U[i * R + b]) : 0.0;
+                    if constexpr (F64)
+                        acc[tt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[tt], 0, 0, 0);
+                    else
+                        acc[tt] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)av, (float)bv, acc[tt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    double* P = part + (size_t)blockIdx.x * R * R;
+#pragma unroll
+    for (int tt = 0; tt < GRAM_MAXT; ++tt) {
+        const int t = wave + 4 * tt;
+        if (t < ntiles) {
+            const int b = (t % T16) * 16 + r16;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                // v_mfma_f64_16x16x4f64: lane l, register v holds row l/16 + 4v;
+                // v_mfma_f32_16x16x4f32: row 4*(l/16) + v (both column l%16)
+                const int a = (t / T16) * 16 + (F64 ? kq + 4 * v : 4 * kq + v);
+                if (a < R && b < R) P[a * R + b] = (double)acc[tt][v];
+            }
+        }
+    }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ part, int nparts, int n, double* __restrict__ out)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int p = 0; p < nparts; ++p) s += part[(size_t)p * n + k];
+        out[k] = s;
+    }
+}
+
+int gram_blocks(uint64_t I)
+{
+    uint64_t b = (I + GRAM_ROWS - 1) / GRAM_ROWS;
+    if (b > 512) b = 512;
+    return b < 1 ? 1 : (int)b;
+}
+
+template <typename T>
+int gram_device(const T* U, uint64_t I, int R, double* G, double* part, hipStream_t s, bool f64)
+{
+    if (R > 128) {
+        set_error("gram: rank above 128");
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    int nb = gram_blocks(I);
+    if (f64)
+        hipLaunchKernelGGL((k_gram_partial<T, true>), dim3(nb), dim3(256), 0, s, U, I, R, part);
+    else
+        hipLaunchKernelGGL((k_gram_partial<T, false>), dim3(nb), dim3(256), 0, s, U, I, R, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3((R * R + 255) / 256), dim3(256), 0, s, part, nb, R * R, G);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+template int gram_device<double>(const double*, uint64_t, int, double*, double*, hipStream_t, bool);
+template int gram_device<float>(const float*, uint64_t, int, double*, double*, hipStream_t, bool);
+template int gram_device<bf16_t>(const bf16_t*, uint64_t, int, double*, double*, hipStream_t, bool);
+
+// ------------------------------------------------------------------ V and its Cholesky factor
+// V = prod_{m != mode} G_m (elementwise; all ones for a 1-mode tensor).
+// Right-looking Cholesky of the lower triangle in LDS.  status: 0 =
+// Cholesky (L in Lout), 1 = pseudo-inverse (Pout).
+template <bool LDS>
+__global__ void __launch_bounds__(256) k_chol(const double* __restrict__ grams, int nmode, int mode, int R,
+                                              double* __restrict__ ws, double* __restrict__ Lout, int* __restrict__ status)
+{
+    extern __shared__ double s_A[];
+    double* A = LDS ? s_A : ws;
+    __shared__ int ok;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < R * R; k += blockDim.x) {
+        double v = 1.0;
+        for (int m = 0; m < nmode; ++m)
+            if (m != mode) v *= grams[(size_t)m * R * R + k];
+        A[k] = v;
+    }
+    if (tid == 0) ok = 1;
+    __syncthreads();
+    for (int j = 0; j < R; ++j) {
+        if (tid == 0) {
+            const double d = A[j * R + j];
+            if (!(d > 0.0)) ok = 0;
+            else A[j * R + j] = sqrt(d);
+        }
+        __syncthreads();
+        if (!ok) break;
+        const double djj = A[j * R + j];
+        for (int i = j + 1 + tid; i < R; i += blockDim.x) A[i * R + j] /= djj;
+        __syncthreads();
+        const int w = R - 1 - j;  // trailing size
+        for (int idx = tid; idx < w * w; idx += blockDim.x) {
+            const int i = j + 1 + idx / w, k = j + 1 + idx % w;
+            if (k <= i) A[i * R + k] -= A[i * R + j] * A[k * R + j];
+        }
+        __syncthreads();
+    }
+    if (ok) {
+        for (int k = tid; k < R * R; k += blockDim.x) {
+            const int i = k / R, c = k % R;
+            Lout[k] = c <= i ? A[k] : 0.0;
+        }
+    }
+    if (tid == 0) *status = ok ? 0 : 1;
+}
+
+// Minimum-norm pseudo-inverse V^+ = Q diag(1/e) Q^T over |e| > R*eps*max|e|
+// (cyclic Jacobi; runs only after a failed Cholesky).  A and Q live in the
+// workspace (global memory), shared through work-group barriers.
+__global__ void __launch_bounds__(256) k_pinv(const double* __restrict__ grams, int nmode, int mode, int R,
+                                              double* __restrict__ A, double* __restrict__ Q, double* __restrict__ Pout,
+                                              const int* __restrict__ status)
+{
+    if (*status == 0) return;
+    const int tid = threadIdx.x;
+    __shared__ double s_red[256];
+    __shared__ double cs[2];
+    __shared__ int s_stop;
+    for (int k = tid; k < R * R; k += blockDim.x) {
+        const int i = k / R, c = k % R;
+        const int lo = i >= c ? k : c * R + i;  // symmetric from the lower triangle
+        double v = 1.0;
+        for (int m = 0; m < nmode; ++m)
+            if (m != mode) v *= grams[(size_t)m * R * R + lo];
+        A[k] = v;
+        Q[k] = i == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0, tot = 0;
+        for (int k = tid; k < R * R; k += blockDim.x) {
+            const double v = A[k] * A[k];
+            tot += v;
+            if (k / R != k % R) off += v;
+        }
+        s_red[tid] = off;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) s_red[tid] += s_red[tid + o];
+            __syncthreads();
+        }
+        const double offs = s_red[0];
+        __syncthreads();
+        s_red[tid] = tot;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) s_red[tid] += s_red[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) s_stop = (offs <= 1e-30 * s_red[0]) || offs == 0.0;
+        __syncthreads();
+        if (s_stop) break;
+        for (int p = 0; p < R - 1; ++p)
+            for (int q = p + 1; q < R; ++q) {
+                if (tid == 0) {
+                    const double apq = A[p * R + q];
+                    if (fabs(apq) < 1e-300) {
+                        cs[0] = 1.0;
+                        cs[1] = 0.0;
+                    } else {
+                        const double th = (A[q * R + q] - A[p * R + p]) / (2.0 * apq);
+                        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                        cs[0] = 1.0 / sqrt(t * t + 1.0);
+                        cs[1] = t * cs[0];
+                    }
+                }
+                __syncthreads();
+                const double c = cs[0], s = cs[1];
+                if (s != 0.0) {
+                    for (int k = tid; k < R; k += blockDim.x) {
+                        const double akp = A[k * R + p], akq = A[k * R + q];
+                        A[k * R + p] = c * akp - s * akq;
+                        A[k * R + q] = s * akp + c * akq;
+                    }
+                    __syncthreads();
+                    for (int k = tid; k < R; k += blockDim.x) {
+                        const double apk = A[p * R + k], aqk = A[q * R + k];
+                        A[p * R + k] = c * apk - s * aqk;
+                        A[q * R + k] = s * apk + c * aqk;
+                        const double qkp = Q[k * R + p], qkq = Q[k * R + q];
+                        Q[k * R + p] = c * qkp - s * qkq;
+                        Q[k * R + q] = s * qkp + c * qkq;
+                    }
+                }
+                __syncthreads();
+            }
+    }
+    double emax = 0;
+    for (int k = 0; k < R; ++k) emax = fmax(emax, fabs(A[k * R + k]));
+    const double tol = emax * R * 2.220446049250313e-16;
+    for (int k = tid; k < R * R; k += blockDim.x) {
+        const int i = k / R, j = k % R;
+        double v = 0;
+        for (int e = 0; e < R; ++e) {
+            const double ev = A[e * R + e];
+            if (fabs(ev) > tol) v += Q[i * R + e] * Q[j * R + e] / ev;
+        }
+        Pout[k] = v;
+    }
+}
+
+// ------------------------------------------------------------------ row solves
+// One lane group per row, column c = gl + q*G held by lane gl.  Cholesky path:
+// forward then backward substitution (POTRS); fallback: x = V^+ u.  Writes
+// the FP64/FP32 master row and the per-block column sums of squares / max.
+template <typename MT, typename UT, int G, int E>
+__global__ void __launch_bounds__(256) k_solve(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ L,
+                                               const double* __restrict__ P, const int* __restrict__ status,
+                                               UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax)
+{
+    const int gl = threadIdx.x & (G - 1);
+    const int gid = threadIdx.x / G;
+    constexpr int GPB = 256 / G;
+    const bool chol = *status == 0;
+    double ss[E], mx[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        ss[q] = 0;
+        mx[q] = -INFINITY;
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * GPB + gid; i < I; i += (uint64_t)gridDim.x * GPB) {
+        double t[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int c = gl + q * G;
+            t[q] = c < R ? (double)M[i * R + c] : 0.0;
+        }
+        if (chol) {
+            for (int k = 0; k < R; ++k) {  // L y = u
+                const int owner = k % G, qk = k / G;
+                double tk = 0;
+#pragma unroll
+                for (int q = 0; q < E; ++q)
+                    if (q == qk) tk = t[q];
+                const double yk = __shfl(tk, owner, G) / L[k * R + k];
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    const int c = gl + q * G;
+                    if (c == k) t[q] = yk;
+                    else if (c > k && c < R) t[q] -= L[c * R + k] * yk;
+                }
+            }
+            for (int k = R - 1; k >= 0; --k) {  // L^T x = y
+                const int owner = k % G, qk = k / G;
+                double tk = 0;
+#pragma unroll
+                for (int q = 0; q < E; ++q)
+                    if (q == qk) tk = t[q];
+                const double xk = __shfl(tk, owner, G) / L[k * R + k];
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    const int c = gl + q * G;
+                    if (c == k) t[q] = xk;
+                    else if (c < k) t[q] -= L[k * R + c] * xk;
+                }
+            }
+        } else {
+            double x[E];
+#pragma unroll
+            for (int q = 0; q < E; ++q) x[q] = 0;
+            for (int k = 0; k < R; ++k) {
+                const int owner = k % G, qk = k / G;
+                double tk = 0;
+#pragma unroll
+                for (int q = 0; q < E; ++q)
+                    if (q == qk) tk = t[q];
+                const double uk = __shfl(tk, owner, G);
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    const int c = gl + q * G;
+                    if (c < R) x[q] += P[c * R + k] * uk;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < E; ++q) t[q] = x[q];
+        }
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int c = gl + q * G;
+            if (c < R) {
+                U[i * R + c] = (UT)t[q];
+                const double v = (double)(UT)t[q];
+                ss[q] += v * v;
+                mx[q] = fmax(mx[q], v);
+            }
+        }
+    }
+    // block reduction of the column statistics: groups combine through LDS
+    __shared__ double s_ss[256 * E], s_mx[256 * E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        s_ss[threadIdx.x * E + q] = ss[q];
+        s_mx[threadIdx.x * E + q] = mx[q];
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < R; c += blockDim.x) {
+        const int l = c % G, q = c / G;
+        double a = 0, b = -INFINITY;
+        for (int g = 0; g < GPB; ++g) {
+            a += s_ss[(g * G + l) * E + q];
+            b = fmax(b, s_mx[(g * G + l) * E + q]);
+        }
+        psum[(size_t)blockIdx.x * R + c] = a;
+        pmax[(size_t)blockIdx.x * R + c] = b;
+    }
+}
+
+// lambda: iteration 0 -> 2-norm of each column; later -> max(1, max_i U[i,r])
+__global__ void k_lambda(const double* __restrict__ psum, const double* __restrict__ pmax, int nb, int R, int which,
+                         double* __restrict__ lambda)
+{
+    for (int c = threadIdx.x; c < R; c += blockDim.x) {
+        double a = 0, b = -INFINITY;
+        for (int p = 0; p < nb; ++p) {
+            a += psum[(size_t)p * R + c];
+            b = fmax(b, pmax[(size_t)p * R + c]);
+        }
+        lambda[c] = which == 0 ? sqrt(a) : fmax(1.0, fmax(b, 0.0));
+    }
+}
+
+// U /= lambda (per column), with the storage copy used by MTTKRP
+template <typename UT, typename ST>
+__global__ void k_scale_cols(UT* __restrict__ U, uint64_t n, int R, const double* __restrict__ lambda, ST* __restrict__ S)
+{
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x) {
+        const double v = (double)U[k] / lambda[k % R];
+        U[k] = (UT)v;
+        if (S) S[k] = store_as<ST>(v);
+    }
+}
+
+// fit inner-product partials: sum_i U[i,r] * M[i,r]; thread t covers
+// column t % R of rows t / R + k * (256 / R) (coalesced)
+template <typename UT, typename MT>
+__global__ void k_inner_partial(const UT* __restrict__ U, const MT* __restrict__ Mm, uint64_t I, int R,
+                                double* __restrict__ part)
+{
+    __shared__ double s[256];
+    const int RP = 256 / R;
+    const int t = threadIdx.x;
+    double a = 0;
+    if (t < RP * R) {
+        const int c = t % R, ri = t / R;
+        for (uint64_t i = (uint64_t)blockIdx.x * RP + ri; i < I; i += (uint64_t)gridDim.x * RP)
+            a += (double)U[i * R + c] * (double)Mm[i * R + c];
+    }
+    s[t] = a;
+    __syncthreads();
+    for (int c = t; c < R; c += blockDim.x) {
+        double v = 0;
+        for (int ri = 0; ri < RP; ++ri) v += s[ri * R + c];
+        part[(size_t)blockIdx.x * R + c] = v;
+    }
+}
+
+// fit = 1 - sqrt(res)/sqrt(normX), res = normX + normU - 2 inner (sqrt only
+// when res > 0, as in the reference)
+__global__ void k_fit(const double* __restrict__ grams, int nmode, int R, const double* __restrict__ lambda,
+                      const double* __restrict__ ipart, int nb, double normX, double* __restrict__ out)
+{
+    __shared__ double s_in[256], s_nu[256];
+    double inner = 0, nu = 0;
+    for (int c = threadIdx.x; c < R; c += blockDim.x) {
+        double a = 0;
+        for (int p = 0; p < nb; ++p) a += ipart[(size_t)p * R + c];
+        inner += a * lambda[c];
+    }
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
+        const int i = k / R, j = k % R;
+        if (j > i) continue;
+        double t = lambda[i] * lambda[j];
+        for (int m = 0; m < nmode; ++m) t *= grams[(size_t)m * R * R + k];
+        nu += i == j ? t : 2.0 * t;
+    }
+    s_in[threadIdx.x] = inner;
+    s_nu[threadIdx.x] = nu;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) {
+            s_in[threadIdx.x] += s_in[threadIdx.x + o];
+            s_nu[threadIdx.x] += s_nu[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double normU = fabs(s_nu[0]);
+        double res = normX + normU - 2.0 * s_in[0];
+        if (res > 0.0) res = sqrt(res);
+        out[0] = 1.0 - res / sqrt(normX);
+        out[1] = s_in[0];
+        out[2] = normU;
+    }
+}
+
+__global__ void k_sumsq_partial(const double* __restrict__ v, uint64_t n, double* __restrict__ part)
+{
+    __shared__ double s[256];
+    double a = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a += v[i] * v[i];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// ------------------------------------------------------------------ host wrappers
+int solve_blocks(uint64_t I, int R)
+{
+    int G = group_lanes(R);
+    uint64_t gpb = 256 / G;
+    uint64_t b = (I + gpb - 1) / gpb;
+    if (b > 1024) b = 1024;
+    return b < 1 ? 1 : (int)b;
+}
+
+int chol_device(const double* grams, int nmode, int mode, int R, double* L, double* P, double* wsA, double* wsQ,
+                int* status, hipStream_t s)
+{
+    if (R > 128) {
+        set_error("pseudo-inverse: rank above 128");
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    if (R <= 64)
+        hipLaunchKernelGGL((k_chol<true>), dim3(1), dim3(256), (size_t)R * R * 8, s, grams, nmode, mode, R, wsA, L, status);
+    else
+        hipLaunchKernelGGL((k_chol<false>), dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, L, status);
+    hipLaunchKernelGGL(k_pinv, dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, wsQ, P, status);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+template <typename MT, typename UT>
+int solve_device(const MT* M, uint64_t I, int R, const double* L, const double* P, const int* status, UT* U,
+                 double* psum, double* pmax, int nb, hipStream_t s)
+{
+    RANK_DISPATCH(R, 1, {
+        (void)PW;
+        hipLaunchKernelGGL((k_solve<MT, UT, G, E>), dim3(nb), dim3(256), 0, s, M, I, R, L, P, status, U, psum, pmax);
+    });
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+template int solve_device<double, double>(const double*, uint64_t, int, const double*, const double*, const int*,
+                                          double*, double*, double*, int, hipStream_t);
+template int solve_device<float, float>(const float*, uint64_t, int, const double*, const double*, const int*, float*,
+                                        double*, double*, int, hipStream_t);
+
+int lambda_device(const double* psum, const double* pmax, int nb, int R, int which, double* lambda, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(128), 0, s, psum, pmax, nb, R, which, lambda);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+template <typename UT, typename ST>
+int scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, hipStream_t s)
+{
+    uint64_t n = I * (uint64_t)R;
+    uint64_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL((k_scale_cols<UT, ST>), dim3((unsigned)(b ? b : 1)), dim3(256), 0, s, U, n, R, lambda, S);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+template int scale_device<double, double>(double*, uint64_t, int, const double*, double*, hipStream_t);
+template int scale_device<float, float>(float*, uint64_t, int, const double*, float*, hipStream_t);
+template int scale_device<float, bf16_t>(float*, uint64_t, int, const double*, bf16_t*, hipStream_t);
+
+constexpr int INNER_BLOCKS = 64;
+
+template <typename UT, typename MT>
+int inner_partial_device(const UT* U, const MT* Mm, uint64_t I, int R, double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_inner_partial<UT, MT>), dim3(INNER_BLOCKS), dim3(256), 0, s, U, Mm, I, R, part);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+template int inner_partial_device<double, double>(const double*, const double*, uint64_t, int, double*, hipStream_t);
+template int inner_partial_device<float, float>(const float*, const float*, uint64_t, int, double*, hipStream_t);
+
+int fit_device(const double* grams, int nmode, int R, const double* lambda, const double* ipart, double normX,
+               double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, grams, nmode, R, lambda, ipart, INNER_BLOCKS, normX, out);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+int normsq_device(const double* v, uint64_t n, double* part, double* out, hipStream_t s)
+{
+    const int nb = 256;
+    hipLaunchKernelGGL(k_sumsq_partial, dim3(nb), dim3(256), 0, s, v, n, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1), 0, s, part, nb, 1, out);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+}  // namespace alto

@@ -28,6 +28,11 @@ __device__ inline bf16_t to_bf16(float f)
     return r;
 }
 
+template <typename ST> __device__ inline ST store_as(double v);
+template <> __device__ inline double store_as<double>(double v) { return v; }
+template <> __device__ inline float store_as<float>(double v) { return (float)v; }
+template <> __device__ inline bf16_t store_as<bf16_t>(double v) { return to_bf16((float)v); }
+
 template <int PW>
 __device__ inline uint64_t field(const uint64_t (&pk)[PW], int off, int bits)
 {
@@ -63,13 +68,14 @@ __device__ inline T group_sum(T v)
 __device__ inline void atomic_add_hw(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ inline void atomic_add_hw(float* p, float v) { unsafeAtomicAdd(p, v); }
 
-template <typename AT, typename OT, int E, int G>
-__device__ inline void flush_row(OT* out, uint64_t row, int R, int gl, const AT (&acc)[E])
+template <typename AT, typename OT, int G, int E>
+__device__ inline void flush_row(OT* out, uint32_t row, int R, int gl, const AT (&acc)[E])
 {
+    OT* o = out + (uint64_t)row * R;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int c = gl + q * G;
-        if (c < R) atomic_add_hw(&out[row * (uint64_t)R + c], (OT)acc[q]);
+        if (c < R) atomic_add_hw(&o[c], (OT)acc[q]);
     }
 }
 
@@ -87,13 +93,15 @@ template <typename FT, typename VT, typename OT>
 struct StreamArgs {
     const uint64_t* pk;
     const VT* val;
-    const FT* U[ALTO_MAXN];
+    const FT* Uo[ALTO_MAXN - 1];      // factors of the other modes
+    int offo[ALTO_MAXN - 1], bitso[ALTO_MAXN - 1];
+    int nother;
+    int offn, bitsn;                   // the target mode's field
     OT* out;
     uint64_t nnz;
     uint64_t seg;
     uint64_t nblocks_real;
     int nmode, mode, rank;
-    int off[ALTO_MAXN], bits[ALTO_MAXN];
 };
 
 template <typename FT, typename VT, typename OT>
@@ -102,11 +110,13 @@ inline void fill_stream_args(const alto_tensor* t, int mode, int R, int seg, Str
     a.pk = t->ms[mode].pk.as<uint64_t>();
     a.val = nullptr;
     a.out = nullptr;
-    for (int m = 0; m < ALTO_MAXN; ++m) {
-        a.U[m] = nullptr;
-        a.off[m] = m < t->nmode ? t->pack.off[m] : 0;
-        a.bits[m] = m < t->nmode ? t->pack.bits[m] : 0;
+    for (int o = 0; o < ALTO_MAXN - 1; ++o) {
+        a.Uo[o] = nullptr;
+        a.offo[o] = a.bitso[o] = 0;
     }
+    a.nother = 0;
+    a.offn = t->pack.off[mode];
+    a.bitsn = t->pack.bits[mode];
     a.nnz = t->nnz;
     a.seg = (uint64_t)seg;
     a.nmode = t->nmode;
@@ -114,8 +124,24 @@ inline void fill_stream_args(const alto_tensor* t, int mode, int R, int seg, Str
     a.rank = R;
 }
 
-inline int group_lanes(int R) { return R <= 8 ? 8 : R <= 16 ? 16 : R <= 32 ? 32 : 64; }
-inline int default_seg(int R) { (void)R; return 256; }
+template <typename FT, typename VT, typename OT>
+inline void set_factors(const alto_tensor* t, int mode, const FT* const* U, StreamArgs<FT, VT, OT>& a)
+{
+    int o = 0;
+    for (int m = 0; m < t->nmode; ++m) {
+        if (m == mode) continue;
+        a.Uo[o] = U[m];
+        a.offo[o] = t->pack.off[m];
+        a.bitso[o] = t->pack.bits[m];
+        ++o;
+    }
+    a.nother = o;
+}
+
+// lanes per nonzero: 2 strided columns per lane, 4..64 lanes
+inline int group_lanes(int R) { return R <= 8 ? 4 : R <= 16 ? 8 : R <= 32 ? 16 : R <= 64 ? 32 : 64; }
+int tuned_seg(int R);
+inline int default_seg(int R) { return tuned_seg(R); }
 
 template <typename A>
 inline dim3 stream_grid(uint64_t nnz, int seg, int G, A& a)
@@ -129,24 +155,21 @@ inline dim3 stream_grid(uint64_t nnz, int seg, int G, A& a)
     return dim3((unsigned)(nb8 * 8));
 }
 
-#define RANK_DISPATCH(R_, PW_, ...)                                                      \
-    do {                                                                                 \
-        const int _r = (R_);                                                             \
-        if ((PW_) == 1) {                                                                \
-            constexpr int PW = 1;                                                        \
-            if (_r <= 8) { constexpr int G = 8, E = 1; __VA_ARGS__; }                    \
-            else if (_r <= 16) { constexpr int G = 16, E = 1; __VA_ARGS__; }             \
-            else if (_r <= 32) { constexpr int G = 32, E = 1; __VA_ARGS__; }             \
-            else if (_r <= 64) { constexpr int G = 64, E = 1; __VA_ARGS__; }             \
-            else { constexpr int G = 64, E = 2; __VA_ARGS__; }                           \
-        } else {                                                                         \
-            constexpr int PW = 2;                                                        \
-            if (_r <= 8) { constexpr int G = 8, E = 1; __VA_ARGS__; }                    \
-            else if (_r <= 16) { constexpr int G = 16, E = 1; __VA_ARGS__; }             \
-            else if (_r <= 32) { constexpr int G = 32, E = 1; __VA_ARGS__; }             \
-            else if (_r <= 64) { constexpr int G = 64, E = 1; __VA_ARGS__; }             \
-            else { constexpr int G = 64, E = 2; __VA_ARGS__; }                           \
-        }                                                                                \
+#define RANK_DISPATCH_PW(PW_, ...)                                                    \
+    do {                                                                               \
+        if ((PW_) == 1) { constexpr int PW = 1; __VA_ARGS__; }                         \
+        else { constexpr int PW = 2; __VA_ARGS__; }                                    \
+    } while (0)
+
+#define RANK_DISPATCH(R_, PW_, ...)                                                    \
+    do {                                                                               \
+        const int _r = (R_);                                                           \
+        constexpr int E = 2;                                                           \
+        if (_r <= 8) { constexpr int G = 4; RANK_DISPATCH_PW(PW_, __VA_ARGS__); }      \
+        else if (_r <= 16) { constexpr int G = 8; RANK_DISPATCH_PW(PW_, __VA_ARGS__); }\
+        else if (_r <= 32) { constexpr int G = 16; RANK_DISPATCH_PW(PW_, __VA_ARGS__); }\
+        else if (_r <= 64) { constexpr int G = 32; RANK_DISPATCH_PW(PW_, __VA_ARGS__); }\
+        else { constexpr int G = 64; RANK_DISPATCH_PW(PW_, __VA_ARGS__); }             \
     } while (0)
 
 template <typename T>

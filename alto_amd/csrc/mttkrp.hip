@@ -20,7 +20,42 @@
 
 namespace alto {
 
-template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW>
+// nonzeros per lane group; set by alto_tune_set (ALTO_TUNE_SEG) or the
+// autotuner, else 256
+static int g_seg = 0;
+int tuned_seg(int R)
+{
+    (void)R;
+    return g_seg > 0 ? g_seg : 256;
+}
+static int g_chunk = 0;
+int tuned_chunk_log2() { return g_chunk; }
+int set_chunk(int v)
+{
+    if (v < 0 || v > 31) return ALTO_ERR_ARG;
+    g_chunk = v;
+    return ALTO_OK;
+}
+int set_seg(int v)
+{
+    if (v < 0 || v > (1 << 20)) return ALTO_ERR_ARG;
+    g_seg = v;
+    return ALTO_OK;
+}
+
+// Columns are strided over the group: lane gl holds c = gl + q*G, so each
+// gather and each atomic flush instruction touches G contiguous elements.
+template <typename FT, typename AT, int G, int E>
+__device__ inline void load_cols(const FT* __restrict__ row, int gl, int R, AT (&x)[E])
+{
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int c = gl + q * G;
+        x[q] = c < R ? to_acc<AT>(row[c]) : (AT)0;
+    }
+}
+
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J>
 __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
 {
     const int lane = threadIdx.x & 63;
@@ -32,60 +67,83 @@ __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
     if (s >= a.nnz) return;
     const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
     const int R = a.rank;
-    const int mode = a.mode;
+    const int nother = a.nother;
 
     AT acc[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) acc[q] = 0;
-    uint64_t cur = ~0ull;
+    uint32_t cur = 0xffffffffu;
 
     for (uint64_t p = s; p < e; p += G) {
-        uint64_t myk[PW];
+        // phase 0: lane gl decodes nonzero p+gl once (coalesced load)
+        uint32_t offs[ALTO_MAXN - 1];
+        uint32_t rown = 0;
         VT myv = 0;
-        if (p + gl < e) {
+        {
+            uint64_t k[PW];
+            if (p + gl < e) {
 #pragma unroll
-            for (int w = 0; w < PW; ++w) myk[w] = a.pk[(p + gl) * PW + w];
-            myv = a.val[p + gl];
-        } else {
+                for (int w = 0; w < PW; ++w) k[w] = a.pk[(p + gl) * PW + w];
+                myv = a.val[p + gl];
+            } else {
 #pragma unroll
-            for (int w = 0; w < PW; ++w) myk[w] = 0;
+                for (int w = 0; w < PW; ++w) k[w] = 0;
+            }
+#pragma unroll
+            for (int o = 0; o < ALTO_MAXN - 1; ++o)
+                offs[o] = o < nother ? (uint32_t)(field<PW>(k, a.offo[o], a.bitso[o]) * (uint64_t)R) : 0u;
+            rown = (uint32_t)field<PW>(k, a.offn, a.bitsn);
         }
         const int cnt = (int)(e - p < (uint64_t)G ? e - p : (uint64_t)G);
-        for (int j = 0; j < cnt; ++j) {
-            uint64_t k[PW];
+        for (int j0 = 0; j0 < cnt; j0 += J) {
+            // phase 1: broadcast J nonzeros, issue all their row gathers
+            AT prod[J][E];
+            uint32_t rj[J];
 #pragma unroll
-            for (int w = 0; w < PW; ++w) k[w] = shfl_u64(myk[w], j, G);
-            const AT v = (AT)shfl_val(myv, j, G);
-            const uint64_t row = field<PW>(k, a.off[mode], a.bits[mode]);
-            if (row != cur) {
-                if (cur != ~0ull) flush_row<AT, OT, E, G>(a.out, cur, R, gl, acc);
-                cur = row;
+            for (int jj = 0; jj < J; ++jj) {
+                const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+                rj[jj] = (uint32_t)__shfl((int)rown, j, G);
+                const AT v = (AT)shfl_val(myv, j, G);
 #pragma unroll
-                for (int q = 0; q < E; ++q) acc[q] = 0;
+                for (int q = 0; q < E; ++q) prod[jj][q] = v;
             }
-            AT prod[E];
 #pragma unroll
-            for (int q = 0; q < E; ++q) prod[q] = v;
-            for (int m = 0; m < a.nmode; ++m) {
-                if (m == mode) continue;
-                const uint64_t r = field<PW>(k, a.off[m], a.bits[m]);
-                const FT* u = a.U[m] + r * (uint64_t)R;
+            for (int o = 0; o < ALTO_MAXN - 1; ++o) {
+                if (o < nother) {
+                    const FT* __restrict__ U = a.Uo[o];
 #pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    const int c = gl + q * G;
-                    prod[q] *= c < R ? to_acc<AT>(u[c]) : (AT)0;
+                    for (int jj = 0; jj < J; ++jj) {
+                        const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+                        const uint32_t off = (uint32_t)__shfl((int)offs[o], j, G);
+                        AT x[E];
+                        load_cols<FT, AT, G, E>(U + off, gl, R, x);
+#pragma unroll
+                        for (int q = 0; q < E; ++q) prod[jj][q] *= x[q];
+                    }
                 }
             }
+            // phase 2: register reduction while the output row repeats
 #pragma unroll
-            for (int q = 0; q < E; ++q) acc[q] += prod[q];
+            for (int jj = 0; jj < J; ++jj) {
+                if (j0 + jj < cnt) {
+                    if (rj[jj] != cur) {
+                        if (cur != 0xffffffffu) flush_row<AT, OT, G, E>(a.out, cur, R, gl, acc);
+                        cur = rj[jj];
+#pragma unroll
+                        for (int q = 0; q < E; ++q) acc[q] = 0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < E; ++q) acc[q] += prod[jj][q];
+                }
+            }
         }
     }
-    if (cur != ~0ull) flush_row<AT, OT, E, G>(a.out, cur, R, gl, acc);
+    if (cur != 0xffffffffu) flush_row<AT, OT, G, E>(a.out, cur, R, gl, acc);
 }
 
 // Phi: per nonzero pi = prod_{m != n} U_m[row_m,:], d = <B[row,:], pi>,
-// contribution (x / max(d, eps)) * pi.  B's row is reloaded on row change.
-template <int G, int E, int PW>
+// contribution (x / max(d, eps)) * pi.
+template <int G, int E, int PW, int J>
 __global__ void __launch_bounds__(256) k_phi(StreamArgs<double, double, double> a, const double* __restrict__ B, double eps)
 {
     const int lane = threadIdx.x & 63;
@@ -97,63 +155,80 @@ __global__ void __launch_bounds__(256) k_phi(StreamArgs<double, double, double> 
     if (s >= a.nnz) return;
     const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
     const int R = a.rank;
-    const int mode = a.mode;
+    const int nother = a.nother;
 
-    double acc[E], b[E];
+    double acc[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) acc[q] = b[q] = 0;
-    uint64_t cur = ~0ull;
+    for (int q = 0; q < E; ++q) acc[q] = 0;
+    uint32_t cur = 0xffffffffu;
     for (uint64_t p = s; p < e; p += G) {
-        uint64_t myk[PW];
+        uint32_t offs[ALTO_MAXN - 1];
+        uint32_t rown = 0;
         double myv = 0;
-        if (p + gl < e) {
+        {
+            uint64_t k[PW];
+            if (p + gl < e) {
 #pragma unroll
-            for (int w = 0; w < PW; ++w) myk[w] = a.pk[(p + gl) * PW + w];
-            myv = a.val[p + gl];
-        } else {
+                for (int w = 0; w < PW; ++w) k[w] = a.pk[(p + gl) * PW + w];
+                myv = a.val[p + gl];
+            } else {
 #pragma unroll
-            for (int w = 0; w < PW; ++w) myk[w] = 0;
+                for (int w = 0; w < PW; ++w) k[w] = 0;
+            }
+#pragma unroll
+            for (int o = 0; o < ALTO_MAXN - 1; ++o)
+                offs[o] = o < nother ? (uint32_t)(field<PW>(k, a.offo[o], a.bitso[o]) * (uint64_t)R) : 0u;
+            rown = (uint32_t)field<PW>(k, a.offn, a.bitsn);
         }
         const int cnt = (int)(e - p < (uint64_t)G ? e - p : (uint64_t)G);
-        for (int j = 0; j < cnt; ++j) {
-            uint64_t k[PW];
+        for (int j0 = 0; j0 < cnt; j0 += J) {
+            double pi[J][E], b[J][E], v[J];
+            uint32_t rj[J];
 #pragma unroll
-            for (int w = 0; w < PW; ++w) k[w] = shfl_u64(myk[w], j, G);
-            const double v = shfl_val(myv, j, G);
-            const uint64_t row = field<PW>(k, a.off[mode], a.bits[mode]);
-            if (row != cur) {
-                if (cur != ~0ull) flush_row<double, double, E, G>(a.out, cur, R, gl, acc);
-                cur = row;
+            for (int jj = 0; jj < J; ++jj) {
+                const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+                rj[jj] = (uint32_t)__shfl((int)rown, j, G);
+                v[jj] = shfl_val(myv, j, G);
+                load_cols<double, double, G, E>(B + (uint64_t)rj[jj] * R, gl, R, b[jj]);
 #pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    acc[q] = 0;
-                    const int c = gl + q * G;
-                    b[q] = c < R ? B[row * (uint64_t)R + c] : 0.0;
+                for (int q = 0; q < E; ++q) pi[jj][q] = 1.0;
+            }
+#pragma unroll
+            for (int o = 0; o < ALTO_MAXN - 1; ++o) {
+                if (o < nother) {
+                    const double* __restrict__ U = a.Uo[o];
+#pragma unroll
+                    for (int jj = 0; jj < J; ++jj) {
+                        const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+                        const uint32_t off = (uint32_t)__shfl((int)offs[o], j, G);
+                        double x[E];
+                        load_cols<double, double, G, E>(U + off, gl, R, x);
+#pragma unroll
+                        for (int q = 0; q < E; ++q) pi[jj][q] *= x[q];
+                    }
                 }
             }
-            double pi[E];
 #pragma unroll
-            for (int q = 0; q < E; ++q) pi[q] = 1.0;
-            for (int m = 0; m < a.nmode; ++m) {
-                if (m == mode) continue;
-                const uint64_t r = field<PW>(k, a.off[m], a.bits[m]);
-                const double* u = a.U[m] + r * (uint64_t)R;
+            for (int jj = 0; jj < J; ++jj) {
+                double d = 0;
 #pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    const int c = gl + q * G;
-                    pi[q] *= c < R ? u[c] : 0.0;
+                for (int q = 0; q < E; ++q) d += b[jj][q] * pi[jj][q];
+                d = group_sum<G>(d);
+                const double w = v[jj] / fmax(d, eps);
+                if (j0 + jj < cnt) {
+                    if (rj[jj] != cur) {
+                        if (cur != 0xffffffffu) flush_row<double, double, G, E>(a.out, cur, R, gl, acc);
+                        cur = rj[jj];
+#pragma unroll
+                        for (int q = 0; q < E; ++q) acc[q] = 0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < E; ++q) acc[q] += w * pi[jj][q];
                 }
             }
-            double d = 0;
-#pragma unroll
-            for (int q = 0; q < E; ++q) d += b[q] * pi[q];
-            d = group_sum<G>(d);
-            const double w = v / fmax(d, eps);
-#pragma unroll
-            for (int q = 0; q < E; ++q) acc[q] += w * pi[q];
         }
     }
-    if (cur != ~0ull) flush_row<double, double, E, G>(a.out, cur, R, gl, acc);
+    if (cur != 0xffffffffu) flush_row<double, double, G, E>(a.out, cur, R, gl, acc);
 }
 
 // ------------------------------------------------------------------ host side
@@ -163,14 +238,26 @@ static int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const*
 {
     StreamArgs<FT, VT, OT> a;
     fill_stream_args(t, mode, R, seg, a);
-    for (int m = 0; m < t->nmode; ++m) a.U[m] = U[m];
+    set_factors(t, mode, U, a);
     a.val = val;
     a.out = out;
     dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
     RANK_DISPATCH(R, t->pack.pw, {
-        hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, G, E, PW>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, G, E, PW, 4>), grid, dim3(256), 0, s, a);
     });
     HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+// factor row offsets (row * rank) are carried as 32-bit values
+static int check_offsets(const alto_tensor* t, int R)
+{
+    for (int m = 0; m < t->nmode; ++m)
+        if (t->dims[m] * (uint64_t)R >= (1ull << 32)) {
+            set_error("mode %d: dims*rank = %llu exceeds 2^32 (32-bit row offsets)", m,
+                      (unsigned long long)(t->dims[m] * (uint64_t)R));
+            return ALTO_ERR_UNSUPPORTED;
+        }
     return ALTO_OK;
 }
 
@@ -181,6 +268,7 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
         set_error("mttkrp: mode %d / rank %d out of range (rank 1..128)", mode, R);
         return ALTO_ERR_ARG;
     }
+    ALTO_TRY(check_offsets(t, R));
     ALTO_TRY(prepare_mode_streams(t, precision == ALTO_PREC_F64, precision != ALTO_PREC_F64, s));
     uint64_t nout = t->dims[mode] * (uint64_t)R;
     if (seg <= 0) seg = default_seg(R);
@@ -206,16 +294,17 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
         set_error("phi: mode %d / rank %d out of range (rank 1..128)", mode, R);
         return ALTO_ERR_ARG;
     }
+    ALTO_TRY(check_offsets(t, R));
     ALTO_TRY(prepare_mode_streams(t, 1, 0, s));
     if (seg <= 0) seg = default_seg(R);
     ALTO_TRY(fill_zero(phi, t->dims[mode] * (uint64_t)R, s));
     StreamArgs<double, double, double> a;
     fill_stream_args(t, mode, R, seg, a);
-    for (int m = 0; m < t->nmode; ++m) a.U[m] = U[m];
+    set_factors(t, mode, U, a);
     a.val = t->ms[mode].val64.as<double>();
     a.out = phi;
     dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
-    RANK_DISPATCH(R, t->pack.pw, { hipLaunchKernelGGL((k_phi<G, E, PW>), grid, dim3(256), 0, s, a, U[mode], eps); });
+    RANK_DISPATCH(R, t->pack.pw, { hipLaunchKernelGGL((k_phi<G, E, PW, 4>), grid, dim3(256), 0, s, a, U[mode], eps); });
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }
@@ -225,6 +314,14 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
 using namespace alto;
 
 extern "C" {
+
+int alto_tune_set(int param, int value)
+{
+    if (param == ALTO_TUNE_SEG) return set_seg(value);
+    if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
+    set_error("alto_tune_set: unknown parameter %d", param);
+    return ALTO_ERR_ARG;
+}
 
 int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* factors, void* out, int precision,
                 int accumulate, alto_stream_t stream)

@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""CP-ALS throughput of the MI355X ALTO engine on one GPU.
+
+Workload: the 3-mode 12092 x 9184 x 28818 tensor with 76.9M unique
+nonzeros, Zipf(1.1) indices over a seeded permutation of each mode and
+lognormal(0,1) values (synthetic, generated on the GPU), FP64 values and
+factors, 64-bit ALTO keys, rank 32.  A step is one CP-ALS iteration (the
+MTTKRP, pseudo-inverse, normalisation and Gram of every mode plus the fit)
+with the tensor and model resident in HBM.  Printed: one JSON line with
+iterations/s, the all-mode MTTKRP roofline (algorithmic bytes / measured
+kernel time vs 8 TB/s), FP32 / BF16 variants, and the CPU oracle's
+iterations/s on the same tensor (bounded sample: one iteration).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DIMS = [12092, 9184, 28818]
+NNZ = 76_900_000
+RANK = 32
+SEED = 2
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nnz", type=int, default=NNZ)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-variants", action="store_true", help="skip the FP32/BF16 runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the multi-GPU "
+              "path is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import alto_amd as A
+    from alto_amd import gen
+
+    A.lib()
+    stream = torch.cuda.current_stream().cuda_stream
+    nnz = args.nnz
+
+    t0 = time.time()
+    coords, vals = gen.sparse_coo(DIMS, nnz, SEED, dist="zipf", s=1.1, values="lognormal")
+    torch.cuda.synchronize()
+    log(f"generated {nnz} nonzeros in {time.time() - t0:.2f}s")
+    t0 = time.time()
+    T = A.AltoTensor.from_device(DIMS, [c.data_ptr() for c in coords], vals.data_ptr(), nnz, nprtn=16)
+    create_s = time.time() - t0
+    del coords, vals
+    torch.cuda.empty_cache()
+    log(f"ALTO creation time:   {create_s:f}")
+
+    U0, _ = A.kruskal_random_init(DIMS, RANK, SEED)
+
+    def run_als(prec, steps, warmup):
+        sess = A.CpdSession(T, U0, precision=prec, stream=stream)
+        sess.iterate(warmup)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fits, _ = sess.iterate(steps, epsilon=-1.0)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        _, _, fit, tm = sess.get()
+        del sess
+        return dt, fits, tm
+
+    dt, fits, tm = run_als(A.PREC_F64, args.steps, args.warmup)
+    ms_per_step = dt / args.steps * 1e3
+    log(f"FP64 CP-ALS: {args.steps} iterations in {dt:.3f}s ({ms_per_step:.3f} ms/iter), fit {fits[-1]:.6f}")
+
+    # all-mode MTTKRP roofline: algorithmic bytes per launch / kernel time
+    Uf = [torch.from_numpy(u).cuda() for u in U0]
+    outs = [torch.zeros(d, RANK, dtype=torch.float64, device="cuda") for d in DIMS]
+    uptr = A._ptr_array([u.data_ptr() for u in Uf])
+    reps = 10
+    mode_ms, mode_bytes = [], []
+    for m in range(3):
+        A.check(A.lib().alto_mttkrp(T.h, m, RANK, uptr, outs[m].data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            A.check(A.lib().alto_mttkrp(T.h, m, RANK, uptr, outs[m].data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+        e1.record()
+        torch.cuda.synchronize()
+        mode_ms.append(e0.elapsed_time(e1) / reps)
+        b = nnz * (8 + 8) + sum(DIMS[k] * RANK * 8 for k in range(3) if k != m) + DIMS[m] * RANK * 8
+        mode_bytes.append(b)
+    allmode_ms = sum(mode_ms)
+    achieved = sum(mode_bytes) / (allmode_ms * 1e-3) / 1e9
+    log(f"all-mode MTTKRP {allmode_ms:.3f} ms ({', '.join(f'{x:.3f}' for x in mode_ms)}), {achieved:.1f} GB/s")
+    del Uf, outs
+
+    variants = {}
+    if not args.no_variants:
+        for name, prec in (("fp32", A.PREC_F32), ("bf16", A.PREC_BF16)):
+            vdt, vfits, _ = run_als(prec, args.steps, args.warmup)
+            variants[name] = {"iters_per_s": args.steps / vdt, "ms_per_step": vdt / args.steps * 1e3,
+                              "fit": float(vfits[-1]), "fit_delta_vs_fp64": float(abs(vfits[-1] - fits[-1]))}
+            log(f"{name} CP-ALS: {vdt / args.steps * 1e3:.3f} ms/iter, fit {vfits[-1]:.6f}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(T, U0)
+
+    line = {
+        "metric": "CP-ALS iters/s + MTTKRP HBM GB/s (frac peak); batched jobs/s, p99 latency",
+        "value": args.steps / dt,
+        "unit": "CP-ALS iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Zipf(1.1) indices over seeded permutations, lognormal values, generated on the GPU)",
+        "config": {"workload": "cp-als nell2-shape zipf", "dims": DIMS, "nnz": nnz, "rank": RANK,
+                   "alto_key_bits": 64, "modes": 3, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_mttkrp (all-mode, 3 launches)", "ms_per_launch": mode_ms,
+                     "algorithmic_bytes_per_launch": mode_bytes},
+        "mttkrp_allmode_ms": allmode_ms,
+        "alto_create_s": create_s,
+        "cpd_timers_s": tm,
+        "precision_variants": variants,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(T, U0):
+    """The CPU oracle (C + OpenMP restatement of cpd_alto) on the same tensor,
+    exported from the GPU: one CP-ALS iteration, timed."""
+    import numpy as np
+
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    oracle.lib().or_set_threads(threads)
+    st = T.export()
+    inf = T.info()
+    t0 = time.time()
+    o = oracle.OracleAlto.wrap(DIMS, st["idx"].shape[1], T.nprtn, st["masks"], st["idx"], st["vals"],
+                               st["prtn_ptr"], st["intervals"], st["cr_masks"])
+    U = [u.copy() for u in U0]
+    lam = np.ones(RANK)
+    t1 = time.time()
+    it, fit = o.cpd_als(U, lam, 1, 0.0)
+    dt = time.time() - t1
+    log(f"CPU oracle: 1 CP-ALS iteration in {dt:.2f}s on {threads} threads (wrap {t1 - t0:.1f}s)")
+    return {"value": 1.0 / dt, "unit": "CP-ALS iterations/s", "cores": threads, "kind": "port",
+            "sample": f"1 CP-ALS iteration (rank {RANK}) on the full {inf.nnz}-nonzero tensor, "
+                      f"OpenMP over {threads} threads, nprtn {T.nprtn}"}
+
+
+if __name__ == "__main__":
+    main()

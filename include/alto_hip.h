@@ -106,6 +106,14 @@ int alto_pext_keys(const alto_tensor_t* t, const uint64_t* keys, uint64_t n, uin
 int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* factors, void* out,
                 int precision, int accumulate, alto_stream_t stream);
 
+/* Tuning knobs of the stream kernels (no reference counterpart; the
+ * reference specializes at compile time, config.mk:25-26).  ALTO_TUNE_SEG:
+ * nonzeros walked by one lane group (0 = default).  ALTO_TUNE_CHUNK_LOG2:
+ * log2 of the ALTO chunk inside which each mode's stream is row-ordered
+ * (0 = default); streams are re-ordered on their next use. */
+enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2 };
+int alto_tune_set(int param, int value);
+
 /* ---- CP-ALS --------------------------------------------------------------- */
 
 typedef struct {
@@ -123,9 +131,38 @@ int alto_cpd_als(alto_tensor_t* t, uint64_t rank, int max_iters, double epsilon,
                  double* const* U, double* lambda, double* fit_trace, int* iters,
                  alto_cpd_timers_t* timers, int verbose, alto_stream_t stream);
 
+/* Device-resident CP-ALS model (no reference counterpart: the reference's
+ * cpd_alto keeps its temporaries on the stack of one call).  create uploads
+ * the initial factors (host, double) and builds the Grams; iterate runs up
+ * to max_iters further iterations of cpd_alto's loop (stopping early when
+ * |fit - previous fit| < epsilon after the first iteration) with all work on
+ * `stream`; get downloads factors / lambda (any pointer may be NULL) and the
+ * last fit.  One session per thread at a time. */
+typedef struct alto_cpd_session alto_cpd_session_t;
+int alto_cpd_session_create(alto_tensor_t* t, uint64_t rank, int precision, double* const* U,
+                            alto_stream_t stream, alto_cpd_session_t** out);
+int alto_cpd_session_iterate(alto_cpd_session_t* s, int max_iters, double epsilon, double* fit_trace,
+                             int* iters, int* converged, int verbose);
+int alto_cpd_session_get(alto_cpd_session_t* s, double* const* U, double* lambda, double* fit,
+                         alto_cpd_timers_t* timers);
+int alto_cpd_session_destroy(alto_cpd_session_t* s);
+
 /* cpd_fit_alto (include/cpd.hpp:189-271) for a model given in host memory */
 int alto_cpd_fit(alto_tensor_t* t, uint64_t rank, double* const* U, const double* lambda,
                  double* fit);
+
+/* pseudo_inverse (include/cpd.hpp:546-646): U (device dim x rank, double)
+ * <- U V^-1 with V the Hadamard product of grams[m != mode] (device, nmodes
+ * consecutive rank x rank matrices).  Cholesky; when it fails, the minimum-
+ * norm least-squares solution (*used_fallback = 1). */
+int alto_pseudo_inverse(const double* grams, int nmodes, int mode, uint64_t rank, double* U, uint64_t dim,
+                        int* used_fallback, alto_stream_t stream);
+
+/* update_gram (src/kruskal_model.cpp:11-24): gram (device, rank x rank
+ * double, both triangles) = U^T U for a device factor U (dim x rank) stored
+ * as double / float / bf16 per precision; computed on the matrix cores. */
+int alto_update_gram(const void* U, uint64_t dim, uint64_t rank, int precision, double* gram,
+                     alto_stream_t stream);
 
 /* ---- CP-APR --------------------------------------------------------------- */
 
@@ -148,6 +185,11 @@ int alto_cp_apr(alto_tensor_t* t, uint64_t rank, int max_iters, int max_inner, d
  * B = factors[mode].  factors: nmodes device pointers (double). */
 int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* factors, double* phi,
              double eps_div_zero, alto_stream_t stream);
+
+/* KruskalModelRandomInit (src/kruskal_model.cpp:109-125) on host buffers:
+ * glibc srand(seed) then U[n][i] = rand()/RAND_MAX row-major, lambda = 1. */
+int alto_kruskal_random_init(int nmodes, const uint64_t* dims, uint64_t rank, unsigned seed,
+                             double* const* U, double* lambda);
 
 /* ---- device memory helpers (for callers without their own allocator) ---- */
 int alto_device_malloc(void** p, uint64_t bytes);

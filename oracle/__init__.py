@@ -89,6 +89,7 @@ def lib():
         L.or_normsq.restype = C.c_double
         L.or_normsq.argtypes = [C.c_void_p]
         L.or_omp_threads.restype = C.c_int
+        L.or_set_threads.argtypes = [C.c_int]
         _LIB = L
     return _LIB
 

@@ -427,6 +427,7 @@ double or_normsq(const or_alto *at)
 }
 
 int or_omp_threads(void) { return omp_get_max_threads(); }
+void or_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
 /* ------------------------------------------------------------------------ */
 /* MTTKRP variants (include/alto.hpp) */

@@ -109,6 +109,7 @@ int or_cp_apr(or_alto *at, uint64_t rank, int max_iters, int max_inner, double e
 double or_normsq(const or_alto *at);
 
 int or_omp_threads(void);
+void or_set_threads(int n);
 
 #ifdef __cplusplus
 }

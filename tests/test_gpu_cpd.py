@@ -1,0 +1,145 @@
+"""GPU parity of the CP-ALS pieces and driver through the C ABI:
+Gram on MFMA vs numpy, the pseudo-inverse (Cholesky and its minimum-norm
+fallback) vs scipy/LAPACK, and whole CP-ALS runs vs the CPU oracle (FP64:
+fit trace within 1e-10 relative; FP32 / BF16: final fit within 1e-4 / 1e-3
+of the FP64 oracle)."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle
+from tests._tensors import uniform_tensor, zipf_tensor
+
+pytestmark = pytest.mark.gpu
+
+
+def _A():
+    import alto_amd
+    return alto_amd
+
+
+@pytest.mark.parametrize("R", [1, 10, 16, 32, 64, 100])
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_gram_mfma(R, prec):
+    A = _A()
+    rng = np.random.default_rng(R + prec)
+    I = 1000 + 37 * R
+    U = rng.random((I, R)) - 0.3
+    if prec == 0:
+        d, ref = A.DeviceArray.from_numpy(U), U
+    elif prec == 1:
+        d, ref = A.DeviceArray.from_numpy(U.astype(np.float32)), U.astype(np.float32).astype(np.float64)
+    else:
+        b = A.to_bf16_bits(U)
+        d, ref = A.DeviceArray.from_numpy(b), A.from_bf16_bits(b).astype(np.float64)
+    G = A.DeviceArray((R, R), np.float64)
+    A.check(A.lib().alto_update_gram(d.ptr, I, R, prec, G.ptr, None), "gram")
+    want = ref.T @ ref
+    tol = 1e-13 if prec == 0 else 2e-6
+    np.testing.assert_allclose(G.numpy(), want, rtol=tol, atol=tol * np.abs(want).max())
+
+
+def _pinv(grams, mode, M):
+    A = _A()
+    R = M.shape[1]
+    g = A.DeviceArray.from_numpy(np.ascontiguousarray(np.stack(grams)))
+    d = A.DeviceArray.from_numpy(M)
+    fb = ctypes.c_int(-1)
+    A.check(A.lib().alto_pseudo_inverse(g.ptr, len(grams), mode, R, d.ptr, M.shape[0], ctypes.byref(fb), None), "pinv")
+    return d.numpy(), fb.value
+
+
+@pytest.mark.parametrize("R", [5, 16, 32, 64, 100])
+def test_pseudo_inverse_cholesky(R):
+    rng = np.random.default_rng(R)
+    Us = [rng.random((d, R)) for d in (300, 200, 250)]
+    grams = [u.T @ u for u in Us]
+    M = rng.random((200, R))
+    out, fb = _pinv(grams, 1, M)
+    assert fb == 0
+    V = grams[0] * grams[2]
+    want = scipy.linalg.cho_solve(scipy.linalg.cho_factor(V, lower=True), M.T).T
+    np.testing.assert_allclose(out, want, rtol=1e-8, atol=1e-10 * np.abs(want).max())
+
+
+def test_pseudo_inverse_fallback_matches_gelsy():
+    rng = np.random.default_rng(9)
+    R = 8
+    Us = [rng.random((d, R)) for d in (30, 40, 50)]
+    Us[2][:, 5] = 0.0  # V has a zero row/column: Cholesky fails
+    grams = [u.T @ u for u in Us]
+    M = rng.random((40, R))
+    out, fb = _pinv(grams, 1, M)
+    assert fb == 1
+    V = grams[0] * grams[2]
+    want = scipy.linalg.lstsq(V, M.T, lapack_driver="gelsy")[0].T
+    np.testing.assert_allclose(out, want, rtol=1e-8, atol=1e-10)
+
+
+def _als(dims, cidx, vals, R, iters, prec=0, seed=3, eps=0.0):
+    A = _A()
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    U, lam = oracle.random_init(dims, R, seed)
+    n, fit, tm = g.cpd_als(U, lam, iters, eps, prec)
+    return n, fit, U, lam
+
+
+@pytest.mark.parametrize("dims,nnz,R", [([40, 50, 60], 6000, 8), ([30, 20, 10, 40], 8000, 16),
+                                        ([100, 90, 80, 70, 60], 20000, 10), ([200, 300, 250], 30000, 32)])
+def test_cpd_als_fp64_matches_oracle(dims, nnz, R):
+    cidx, vals = uniform_tensor(dims, nnz, 5)
+    n, fit, U, lam = _als(dims, cidx, vals, R, 12)
+    o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
+    Uo, lo = oracle.random_init(dims, R, 3)
+    no, fo = o.cpd_als(Uo, lo, 12, 0.0)
+    assert n == no == 12
+    np.testing.assert_allclose(fit, fo, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(lam, lo, rtol=1e-8)
+    for a, b in zip(U, Uo):
+        np.testing.assert_allclose(a, b, rtol=1e-7, atol=1e-9 * np.abs(b).max())
+
+
+def test_cpd_als_convergence_stop_and_rank1_recovery():
+    dims = [8, 9, 10]
+    rng = np.random.default_rng(12)
+    X = np.einsum("i,j,k->ijk", rng.random(8) + 0.5, rng.random(9) + 0.5, rng.random(10) + 0.5)
+    cidx = np.array(np.nonzero(X), dtype=np.uint64)
+    vals = X[np.nonzero(X)]
+    n, fit, U, lam = _als(dims, cidx, vals, 1, 50, eps=1e-12)
+    o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
+    Uo, lo = oracle.random_init(dims, 1, 3)
+    no, fo = o.cpd_als(Uo, lo, 50, 1e-12)
+    assert fit[-1] > 1 - 1e-9
+    assert n == no < 50
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-4), (2, 1e-3)])
+def test_cpd_als_reduced_precision_fit(prec, tol):
+    dims = [300, 200, 400]
+    cidx, vals = zipf_tensor(dims, 40000, 4, s=1.1)
+    R = 16
+    n, fit, U, lam = _als(dims, cidx, vals, R, 15, prec=prec)
+    o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
+    Uo, lo = oracle.random_init(dims, R, 3)
+    no, fo = o.cpd_als(Uo, lo, 15, 0.0)
+    assert abs(fit[-1] - fo[-1]) <= tol * max(1.0, abs(fo[-1]))
+
+
+def test_session_split_iterations_equal_single_run():
+    A = _A()
+    dims = [60, 70, 80]
+    cidx, vals = uniform_tensor(dims, 8000, 21)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    U, lam = A.kruskal_random_init(dims, 12, 4)
+    Uo, lo = oracle.random_init(dims, 12, 4)
+    for a, b in zip(U, Uo):
+        np.testing.assert_array_equal(a, b)
+    s = A.CpdSession(g, U)
+    f1, _ = s.iterate(3)
+    f2, _ = s.iterate(4)
+    U1, l1, fit, _ = s.get()
+    n, fit_all, U2, l2 = _als(dims, cidx, vals, 12, 7, seed=4)
+    np.testing.assert_allclose(np.concatenate([f1, f2]), fit_all, rtol=1e-13)
+    np.testing.assert_allclose(l1, l2, rtol=1e-12)
