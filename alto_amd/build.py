@@ -44,9 +44,26 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout)
     os.replace(tmp, OUT)
+    build_cli()
     if verbose:
         print(f"built {OUT}")
     return OUT
+
+
+def build_cli() -> None:
+    """cpd64 / cpd128 (the reference command line) linked against the library."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exe = os.path.join(bindir, "cpd64")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), os.path.join(HERE, "cli", "cpd.cpp"),
+           "-o", exe, "-L" + HERE, "-lalto_hip", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cpd CLI build failed:\n" + r.stdout)
+    c128 = os.path.join(bindir, "cpd128")
+    if os.path.lexists(c128):
+        os.remove(c128)
+    os.symlink("cpd64", c128)
 
 
 if __name__ == "__main__":

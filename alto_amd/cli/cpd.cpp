@@ -1,0 +1,315 @@
+// cpd64 / cpd128: the reference's command line (src/main.cpp) over the
+// MI355X engine.  Same flags, defaults and output lines that
+// scripts/run_cpd.sh parses ("ALTO creation time", "ALTO runtime", the
+// "it:" lines and the CP-ALS / CP-APR timer tables).  The binary name picks
+// the ALTO key width (cpd64 / cpd128) as ALTO_MASK_LENGTH does
+// (src/main.cpp:32-39); -l defaults to ALS.
+#include <getopt.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "alto/alto.hpp"
+
+static const char* short_opt = "hvi:o:b:l:r:m:n:x:d:t:s:e:f:zcp";
+static const struct option long_opt[] = {
+    {"help", 0, NULL, 'h'},       {"version", 0, NULL, 'v'},     {"input", 1, NULL, 'i'},
+    {"output", 1, NULL, 'o'},     {"bin", 1, NULL, 'b'},         {"model", 1, NULL, 'l'},
+    {"rank", 1, NULL, 'r'},       {"max-iter", 1, NULL, 'm'},    {"max-inner", 1, NULL, 'n'},
+    {"seed", 1, NULL, 'x'},       {"dims", 1, NULL, 'd'},        {"target-mode", 1, NULL, 't'},
+    {"sparsity", 1, NULL, 's'},   {"epsilon", 1, NULL, 'e'},     {"file", 1, NULL, 'f'},
+    {"zero-based", 0, NULL, 'z'}, {"check", 0, NULL, 'c'},       {"bench", 0, NULL, 'p'},
+    {NULL, 0, NULL, 0}};
+
+static double wtime()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static void usage(const char* call)
+{
+    fprintf(stderr, "Usage: %s [OPTIONS]\n", call);
+    fprintf(stderr, "Options:\n");
+    fprintf(stderr, "\t-h or --help         Display this information\n");
+    fprintf(stderr, "\t-v or --version      Display version information\n");
+    fprintf(stderr, "\t-i or --input        Input tensor file in text\n");
+    fprintf(stderr, "\t-o or --output       Output tensor to file\n");
+    fprintf(stderr, "\t-b or --bin          Input tensor file in binary\n");
+    fprintf(stderr, "\t-z or --zero-based   Assume input tensor is zero-based (default: one-based)\n");
+    fprintf(stderr, "\t-l or --model        CPD model (ALS or APR)\n");
+    fprintf(stderr, "\t-f or --file         Save tensor to another format\n");
+    fprintf(stderr, "\t-r or --rank         Rank\n");
+    fprintf(stderr, "\t-m or --max-iter     Maximum outer iterations\n");
+    fprintf(stderr, "\t-n or --max-inner    Maximum inner iterations\n");
+    fprintf(stderr, "\t-t or --target-mode  Target mode of tensor\n");
+    fprintf(stderr, "\t-e or --epsilon      Convergence criteria\n");
+    fprintf(stderr, "\t-x or --seed         Random value seed\n");
+    fprintf(stderr, "\t-d or --dims         Dimension lenghts (I,J,K)\n");
+    fprintf(stderr, "\t-s or --sparsity     Sparsity of generate tensor\n");
+    fprintf(stderr, "\t-c or --check        Run ALTO (par) validation against cpd MTTKRP\n");
+    fprintf(stderr, "\t-p or --bench        Run ALTO benchmark mode based on the CPD model with the given CMD line options\n");
+}
+
+// .tns text reader (src/sptensor.cpp:75-161): dims are the largest index
+// (+ zero_based), '#' lines and empty lines are skipped, values via strtod.
+static SparseTensor* read_tns(const char* path, int zero_based)
+{
+    FILE* fp = fopen(path, "r");
+    if (!fp) {
+        fprintf(stderr, "cannot open %s\n", path);
+        exit(1);
+    }
+    char* line = NULL;
+    size_t len = 0;
+    ssize_t nread;
+    int nmodes = -1;
+    std::vector<std::vector<IType>> idx;
+    std::vector<FType> vals;
+    while ((nread = getline(&line, &len, fp)) != -1) {
+        if (nread <= 1 || line[0] == '#') continue;
+        if (nmodes < 0) {
+            int tok = 0;
+            char* copy = strdup(line);
+            for (char* p = strtok(copy, " \t\n"); p; p = strtok(NULL, " \t\n")) ++tok;
+            free(copy);
+            nmodes = tok - 1;
+            if (nmodes < 1 || nmodes > MAX_NUM_MODES) {
+                fprintf(stderr, "unsupported number of modes %d\n", nmodes);
+                exit(1);
+            }
+            idx.resize(nmodes);
+        }
+        char* p = line;
+        for (int m = 0; m < nmodes; ++m) idx[m].push_back(strtoull(p, &p, 10) - (zero_based ? 0 : 1));
+        vals.push_back(strtod(p, &p));
+    }
+    free(line);
+    fclose(fp);
+    SparseTensor* X = new SparseTensor();
+    X->nmodes = nmodes;
+    X->nnz = vals.size();
+    X->dims = new IType[nmodes];
+    X->cidx = new IType*[nmodes];
+    for (int m = 0; m < nmodes; ++m) {
+        IType mx = 0;
+        X->cidx[m] = new IType[X->nnz];
+        for (IType i = 0; i < X->nnz; ++i) {
+            X->cidx[m][i] = idx[m][i];
+            if (idx[m][i] + 1 > mx) mx = idx[m][i] + 1;
+        }
+        X->dims[m] = mx;
+    }
+    X->vals = new FType[X->nnz];
+    memcpy(X->vals, vals.data(), X->nnz * sizeof(FType));
+    return X;
+}
+
+// binary reader (src/sptensor.cpp:211-258): nmodes (8 bytes, low 32 bits
+// meaningful), dims, nnz, the index arrays, the values
+static SparseTensor* read_bin(const char* path)
+{
+    FILE* fp = fopen(path, "rb");
+    if (!fp) {
+        fprintf(stderr, "cannot open %s\n", path);
+        exit(1);
+    }
+    IType nm = 0, nnz = 0;
+    if (fread(&nm, 8, 1, fp) != 1) exit(1);
+    nm &= 0xffffffffull;
+    SparseTensor* X = new SparseTensor();
+    X->nmodes = (int)nm;
+    X->dims = new IType[nm];
+    if (fread(X->dims, 8, nm, fp) != nm || fread(&nnz, 8, 1, fp) != 1) exit(1);
+    X->nnz = nnz;
+    X->cidx = new IType*[nm];
+    for (IType m = 0; m < nm; ++m) {
+        X->cidx[m] = new IType[nnz];
+        if (fread(X->cidx[m], 8, nnz, fp) != nnz) exit(1);
+    }
+    X->vals = new FType[nnz];
+    if (fread(X->vals, 8, nnz, fp) != nnz) exit(1);
+    fclose(fp);
+    return X;
+}
+
+static void write_tensor(const char* path, SparseTensor* X, bool binary)
+{
+    FILE* fp = fopen(path, binary ? "wb" : "w");
+    if (!fp) return;
+    if (binary) {
+        IType nm = (IType)X->nmodes;
+        fwrite(&nm, 8, 1, fp);
+        fwrite(X->dims, 8, X->nmodes, fp);
+        fwrite(&X->nnz, 8, 1, fp);
+        for (int m = 0; m < X->nmodes; ++m) fwrite(X->cidx[m], 8, X->nnz, fp);
+        fwrite(X->vals, 8, X->nnz, fp);
+    } else {
+        for (IType i = 0; i < X->nnz; ++i) {
+            for (int m = 0; m < X->nmodes; ++m) fprintf(fp, "%llu ", X->cidx[m][i] + 1);
+            fprintf(fp, "%g\n", X->vals[i]);
+        }
+    }
+    fclose(fp);
+}
+
+static void print_info(IType rank, int max_iters, SparseTensor* X)
+{
+    double tmp = 1;
+    for (int i = 0; i < X->nmodes; i++) tmp *= (double)X->dims[i];
+    fprintf(stdout, "# Modes         = %u\n", X->nmodes);
+    fprintf(stdout, "Rank            = %llu\n", rank);
+    fprintf(stdout, "Sparsity        = %f\n", X->nnz / tmp);
+    fprintf(stdout, "Max iters       = %d\n", max_iters);
+    fprintf(stdout, "Dimensions      = [%llu", X->dims[0]);
+    for (int i = 1; i < X->nmodes; i++) fprintf(stdout, " X %llu", X->dims[i]);
+    fprintf(stdout, "]\n");
+    fprintf(stdout, "NNZ             = %llu\n", X->nnz);
+}
+
+// sequential COO MTTKRP (include/cpd.hpp:509-543): the truth of the -c check
+static void coo_mttkrp(SparseTensor* X, FType** U, IType rank, int mode, FType* out)
+{
+    std::vector<FType> row(rank);
+    for (IType i = 0; i < X->nnz; ++i) {
+        for (IType r = 0; r < rank; ++r) row[r] = X->vals[i];
+        for (int m = 0; m < X->nmodes; ++m)
+            if (m != mode)
+                for (IType r = 0; r < rank; ++r) row[r] *= U[m][X->cidx[m][i] * rank + r];
+        for (IType r = 0; r < rank; ++r) out[X->cidx[mode][i] * rank + r] += row[r];
+    }
+}
+
+template <typename LIT>
+static int run(SparseTensor* X, Model model, IType rank, int max_iters, int max_inner, double epsilon, int seed,
+               int target_mode, bool do_check, bool do_bench, int nprtn)
+{
+    const double eps_div_zero = 1.0e-10, kappa_tol = 1.0e-10, kappa = 1.0e-2;
+    KruskalModel* M;
+    if (do_check) {
+        printf("===Check for mode %d===\n", target_mode);
+        print_info(rank, 1, X);
+        CreateKruskalModel(X->nmodes, X->dims, rank, &M);
+        KruskalModelRandomInit(M, (unsigned)seed);
+        AltoTensor<LIT>* AT;
+        create_alto(X, &AT, nprtn);
+        for (int m = 0; m < X->nmodes; ++m) {
+            if (target_mode != -1 && m != target_mode) continue;
+            std::vector<FType> truth(X->dims[m] * rank, 0.0), saved(M->U[m], M->U[m] + X->dims[m] * rank);
+            coo_mttkrp(X, M->U, rank, m, truth.data());
+            memset(M->U[m], 0, X->dims[m] * rank * sizeof(FType));
+            mttkrp_alto_par(m, M->U, rank, AT, nullptr, nullptr);
+            IType cnt = 0;
+            for (IType k = 0; k < X->dims[m] * rank; ++k)
+                if (fabs(truth[k] - M->U[m][k]) > 1.0e-6) cnt++;
+            printf("mode %d: ", m);
+            fflush(stdout);
+            if (cnt == 0) fprintf(stderr, "Results of mttkrp_alto is correct\n");
+            else fprintf(stderr, "Results of mttkrp_alto is incorrect by %llu\n", cnt);
+            memcpy(M->U[m], saved.data(), saved.size() * sizeof(FType));
+        }
+        destroy_alto(AT);
+        DestroyKruskalModel(M);
+        return 0;
+    }
+    print_info(rank, max_iters, X);
+    CreateKruskalModel(X->nmodes, X->dims, rank, &M);
+    KruskalModelRandomInit(M, (unsigned)seed);
+    AltoTensor<LIT>* AT;
+    double t0 = wtime();
+    create_alto(X, &AT, nprtn);
+    double tc = wtime() - t0;
+    if (do_bench) {
+        printf("ALTO creation time:   %f\n", tc);
+        if (model == APR) {
+            cp_apr_alto(AT, M, max_iters, max_inner, 0.0, 0.0, kappa_tol, kappa, true);
+        } else {
+            alto_dropin::DeviceFactors d(AT->nmode, AT->dims, rank, M->U);
+            auto pass = [&]() {
+                for (int m = 0; m < AT->nmode; ++m)
+                    if (target_mode == -1 || m == target_mode)
+                        alto_dropin::check(alto_mttkrp(AT->dev, m, rank, (const void* const*)d.p.data(), d.p[m],
+                                                       ALTO_PREC_F64, 1, nullptr),
+                                           "mttkrp");
+            };
+            pass();  // warm-up
+            alto_dropin::check(alto_device_synchronize(), "sync");
+            t0 = wtime();
+            for (int i = 0; i < max_iters; ++i) pass();
+            alto_dropin::check(alto_device_synchronize(), "sync");
+            printf("ALTO runtime:   %f\n", wtime() - t0);
+        }
+    } else if (model == ALS) {
+        cpd_alto(AT, M, max_iters, epsilon);
+    } else {
+        cp_apr_alto(AT, M, max_iters, max_inner, epsilon, eps_div_zero, kappa_tol, kappa, false);
+    }
+    destroy_alto(AT);
+    DestroyKruskalModel(M);
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    int max_iters = 10, max_inner = 10, target_mode = -1, save = 0, zero_based = 0;
+    IType rank = 16;
+    double epsilon = 1e-5;
+    int seed = (int)time(NULL);
+    bool do_check = false, do_bench = false;
+    Model model = ALS;
+    std::string text_file, text_out, bin_file, dims_arg;
+    int c;
+    while ((c = getopt_long(argc, argv, short_opt, long_opt, NULL)) != -1) {
+        switch (c) {
+        case 'h': usage(argv[0]); return 0;
+        case 'v': fprintf(stdout, "%s version 0.1.2 (%s)\n", argv[0], alto_version()); return 0;
+        case 'i': text_file = optarg; break;
+        case 'o': text_out = optarg; break;
+        case 'b': bin_file = optarg; break;
+        case 'l':
+            if (!strcmp(optarg, "ALS") || !strcmp(optarg, "als")) model = ALS;
+            else if (!strcmp(optarg, "APR") || !strcmp(optarg, "apr")) model = APR;
+            else { fprintf(stderr, "Invalid -model: %s.\n", optarg); return -1; }
+            break;
+        case 'r': rank = (IType)atoll(optarg); if ((long long)rank < 1) { fprintf(stderr, "Invalid -rank: %s.\n", optarg); return -1; } break;
+        case 'm': max_iters = atoi(optarg); if (max_iters <= 0) { fprintf(stderr, "Invalid -max-iter: %s.\n", optarg); return -1; } break;
+        case 'n': max_inner = atoi(optarg); if (max_inner <= 0) { fprintf(stderr, "Invalid -max-inner: %s.\n", optarg); return -1; } break;
+        case 'x': seed = atoi(optarg); if (seed < 0) { fprintf(stderr, "Invalid -seed: %s.\n", optarg); return -1; } break;
+        case 'd': dims_arg = optarg; break;
+        case 't': target_mode = atoi(optarg); if (target_mode < -1) { fprintf(stderr, "Invalid -target-mode: %s.\n", optarg); return -1; } break;
+        case 's': break;
+        case 'e': epsilon = atof(optarg); if (epsilon <= 0.0) { fprintf(stderr, "Invalid -epsilon: %s.\n", optarg); return -1; } break;
+        case 'f': save = atoi(optarg); if (save < 0) { fprintf(stderr, "Invalid -file: %s.\n", optarg); return -1; } break;
+        case 'z': zero_based = 1; break;
+        case 'c': do_check = true; break;
+        case 'p': do_bench = true; break;
+        default: usage(argv[0]); return -1;
+        }
+    }
+    SparseTensor* X = nullptr;
+    if (!bin_file.empty()) {
+        X = read_bin(bin_file.c_str());
+        if (save) write_tensor(text_out.empty() ? "out.tns" : text_out.c_str(), X, false);
+    } else if (!text_file.empty()) {
+        X = read_tns(text_file.c_str(), zero_based);
+        if (save) write_tensor(text_out.empty() ? "out.bin" : text_out.c_str(), X, true);
+    } else {
+        fprintf(stderr, dims_arg.empty() ? "No dims specified... exiting\n"
+                                         : "-d/-s tensor generation is not part of this build; pass -i or -b\n");
+        usage(argv[0]);
+        return -1;
+    }
+    const char* base = strrchr(argv[0], '/');
+    base = base ? base + 1 : argv[0];
+    const int nprtn = 16;  // reference P = thread count; fixed here for reproducible partitions
+    if (strstr(base, "128"))
+        return run<unsigned __int128>(X, model, rank, max_iters, max_inner, epsilon, seed, target_mode, do_check,
+                                      do_bench, nprtn);
+    return run<unsigned long long>(X, model, rank, max_iters, max_inner, epsilon, seed, target_mode, do_check, do_bench,
+                                   nprtn);
+}

@@ -29,6 +29,7 @@ int tuned_seg(int R)
     return g_seg > 0 ? g_seg : 256;
 }
 static int g_chunk = 0;
+static int g_nt = 0;
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
 {
@@ -55,7 +56,14 @@ __device__ inline void load_cols(const FT* __restrict__ row, int gl, int R, AT (
     }
 }
 
-template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J>
+template <bool NT, typename T>
+__device__ inline T stream_ld(const T* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J, bool NT>
 __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
 {
     const int lane = threadIdx.x & 63;
@@ -83,8 +91,8 @@ __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
             uint64_t k[PW];
             if (p + gl < e) {
 #pragma unroll
-                for (int w = 0; w < PW; ++w) k[w] = a.pk[(p + gl) * PW + w];
-                myv = a.val[p + gl];
+                for (int w = 0; w < PW; ++w) k[w] = stream_ld<NT>(&a.pk[(p + gl) * PW + w]);
+                myv = stream_ld<NT>(&a.val[p + gl]);
             } else {
 #pragma unroll
                 for (int w = 0; w < PW; ++w) k[w] = 0;
@@ -243,7 +251,8 @@ static int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const*
     a.out = out;
     dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
     RANK_DISPATCH(R, t->pack.pw, {
-        hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, G, E, PW, 4>), grid, dim3(256), 0, s, a);
+        if (g_nt) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, G, E, PW, 4, true>), grid, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, G, E, PW, 4, false>), grid, dim3(256), 0, s, a);
     });
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
@@ -319,6 +328,10 @@ int alto_tune_set(int param, int value)
 {
     if (param == ALTO_TUNE_SEG) return set_seg(value);
     if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
+    if (param == ALTO_TUNE_NONTEMPORAL) {
+        g_nt = value ? 1 : 0;
+        return ALTO_OK;
+    }
     set_error("alto_tune_set: unknown parameter %d", param);
     return ALTO_ERR_ARG;
 }

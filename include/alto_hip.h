@@ -110,8 +110,9 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * reference specializes at compile time, config.mk:25-26).  ALTO_TUNE_SEG:
  * nonzeros walked by one lane group (0 = default).  ALTO_TUNE_CHUNK_LOG2:
  * log2 of the ALTO chunk inside which each mode's stream is row-ordered
- * (0 = default); streams are re-ordered on their next use. */
-enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2 };
+ * (0 = default); streams are re-ordered on their next use.
+ * ALTO_TUNE_NONTEMPORAL: 1 = stream the nonzeros with non-temporal loads. */
+enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3 };
 int alto_tune_set(int param, int value);
 
 /* ---- CP-ALS --------------------------------------------------------------- */
