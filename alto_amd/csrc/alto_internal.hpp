@@ -38,6 +38,8 @@ void set_error_hip(hipError_t e, const char* what);
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    size_t cap = 0;   // size class actually reserved
+    int device = 0;
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
@@ -107,7 +109,8 @@ struct alto_tensor {
 namespace alto {
 
 // create.hip
-int build_luts(alto_tensor* t);
+int build_luts(alto_tensor* t, hipStream_t s);
+size_t pool_in_use(int dev);
 int compute_masks(int nmode, const uint64_t* dims, int W, uint64_t* masks, uint64_t* alto_mask);
 int radix_sort_pairs(uint64_t* keys, int W, uint32_t* payload, uint64_t n, int bits, hipStream_t s);
 int delinearize_all(const alto_tensor* t, uint64_t* coords_dev, hipStream_t s);

@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <mutex>
 
@@ -39,26 +40,95 @@ void set_error_hip(hipError_t e, const char* what)
 
 const char* last_error() { return g_err.c_str(); }
 
+// Caching device allocator: freed blocks are kept per device in power-of-two
+// size classes and reused, so concurrent jobs never call hipFree (which
+// synchronizes the device) on their hot path.  alto_device_trim returns the
+// cache to the runtime.
+namespace {
+struct Pool {
+    std::mutex mu;
+    std::multimap<size_t, void*> free_blocks[64];  // per device: class bytes -> ptr
+    size_t cached[64] = {0};
+    size_t in_use[64] = {0};
+};
+Pool& pool()
+{
+    static Pool* p = new Pool();
+    return *p;
+}
+size_t size_class(size_t n)
+{
+    if (n <= 256) return 256;
+    size_t c = 256;
+    while (c < n) c <<= 1;
+    return c;
+}
+}  // namespace
+
 int DevBuf::alloc(size_t n)
 {
     release();
     if (n == 0) n = 8;
-    hipError_t e = hipMalloc(&p, n);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    const size_t cls = size_class(n);
+    Pool& P = pool();
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        auto& fb = P.free_blocks[dev & 63];
+        auto it = fb.find(cls);
+        if (it != fb.end()) {
+            p = it->second;
+            fb.erase(it);
+            P.cached[dev & 63] -= cls;
+            P.in_use[dev & 63] += cls;
+            bytes = n;
+            cap = cls;
+            device = dev;
+            return ALTO_OK;
+        }
+    }
+    hipError_t e = hipMalloc(&p, cls);
+    if (e == hipErrorOutOfMemory) {  // give the cache back and retry once
+        (void)hipGetLastError();
+        alto_device_trim();
+        e = hipMalloc(&p, cls);
+    }
     if (e != hipSuccess) {
         p = nullptr;
-        set_error("hipMalloc(%zu bytes) failed: %s", n, hipGetErrorString(e));
+        set_error("hipMalloc(%zu bytes) failed: %s", cls, hipGetErrorString(e));
         (void)hipGetLastError();
         return e == hipErrorOutOfMemory ? ALTO_ERR_OOM : ALTO_ERR_HIP;
     }
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        P.in_use[dev & 63] += cls;
+    }
     bytes = n;
+    cap = cls;
+    device = dev;
     return ALTO_OK;
 }
 
 void DevBuf::release()
 {
-    if (p) (void)hipFree(p);
+    if (p) {
+        Pool& P = pool();
+        std::lock_guard<std::mutex> g(P.mu);
+        P.free_blocks[device & 63].emplace(cap, p);
+        P.cached[device & 63] += cap;
+        P.in_use[device & 63] -= cap;
+    }
     p = nullptr;
     bytes = 0;
+    cap = 0;
+}
+
+size_t pool_in_use(int dev)
+{
+    Pool& P = pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    return P.in_use[dev & 63];
 }
 
 // ------------------------------------------------------------------ masks
@@ -93,7 +163,7 @@ int compute_masks(int nmode, const uint64_t* dims, int W, uint64_t* masks, uint6
 
 // Packed layout: mode n at bit offset sum_{m<n} bits[m].  The key->packed
 // map is a bit permutation: the j-th set bit of mask_n goes to off[n]+j.
-int build_luts(alto_tensor* t)
+int build_luts(alto_tensor* t, hipStream_t s)
 {
     PackLayout& L = t->pack;
     L.nmode = t->nmode;
@@ -113,7 +183,7 @@ int build_luts(alto_tensor* t)
     for (int p = 0; p < off; ++p) packpos_of[keypos_of[p]] = p;
 
     const int nbits = off;
-    auto fill = [nbits](BitPermLut& lut, int in_w, int out_w, const int* map /* in bit -> out bit or -1 */) -> int {
+    auto fill = [nbits, s](BitPermLut& lut, int in_w, int out_w, const int* map /* in bit -> out bit or -1 */) -> int {
         lut.in_words = in_w;
         lut.out_words = out_w;
         lut.nb = std::min(in_w * 8, (nbits + 7) / 8);
@@ -128,7 +198,8 @@ int build_luts(alto_tensor* t)
                 }
             }
         ALTO_TRY(lut.dev.alloc(lut.host.size() * 8));
-        HIP_TRY(hipMemcpy(lut.dev.p, lut.host.data(), lut.host.size() * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpyAsync(lut.dev.p, lut.host.data(), lut.host.size() * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
         return ALTO_OK;
     };
     int map_delin[128], map_lin[128];
@@ -753,7 +824,7 @@ static int create_impl(int nmode, const uint64_t* dims, uint64_t nnz, const uint
         set_error("alto_tensor_create: %d index bits exceed the %d-bit ALTO mask", total, lit_bits);
         return ALTO_ERR_ARG;
     }
-    ALTO_TRY(build_luts(t));
+    ALTO_TRY(build_luts(t, s));
 
     // inputs to device
     DevBuf dcoord[ALTO_MAXN], dvals;
@@ -1018,6 +1089,18 @@ int alto_memcpy_d2h(void* dst, const void* src, uint64_t bytes)
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return ALTO_OK;
 }
+int alto_device_trim(void)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    Pool& P = pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    for (auto& kv : P.free_blocks[dev & 63]) (void)hipFree(kv.second);
+    P.free_blocks[dev & 63].clear();
+    P.cached[dev & 63] = 0;
+    return ALTO_OK;
+}
+
 int alto_device_synchronize(void)
 {
     HIP_TRY(hipDeviceSynchronize());

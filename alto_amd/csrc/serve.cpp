@@ -1,0 +1,280 @@
+// Continuous-batching CP-ALS server (no reference counterpart: the reference
+// runs one decomposition per process).  Jobs (a COO tensor, rank,
+// iteration limit, tolerance, seed, precision) are submitted at any time;
+// an admission loop admits them in arrival order as soon as a worker slot is
+// free and their estimated HBM footprint fits the budget (a later, smaller
+// job may pass a head job that does not fit yet).  Each worker owns a
+// non-blocking HIP stream and runs create_alto + CP-ALS for its job, so the
+// work of all admitted jobs overlaps on the device; finished jobs release
+// their footprint and the next arrivals are admitted immediately.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "alto_hip.h"
+
+namespace {
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Job {
+    uint64_t id = 0;
+    int nmodes = 0;
+    std::vector<uint64_t> dims;
+    uint64_t nnz = 0;
+    std::vector<std::vector<uint64_t>> cidx;
+    std::vector<double> vals;
+    std::vector<const uint64_t*> dev_cidx;  // device inputs, used in place
+    const double* dev_vals = nullptr;
+    uint64_t rank = 0;
+    int max_iters = 0;
+    double epsilon = 0;
+    unsigned seed = 0;
+    int precision = 0;
+    uint64_t footprint = 0;
+    // results
+    int status = 0;  // 0 queued, 1 running, 2 done, < 0 error code
+    int iters = 0;
+    double fit = 0;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    std::vector<std::vector<double>> U;
+    std::vector<double> lambda;
+    std::string error;
+};
+
+// bytes the job holds on the device: ALTO keys + values, one stream per mode
+// (packed coordinates + value in the job's precision), factors and the
+// per-mode / per-iteration work buffers
+uint64_t footprint_of(const Job& j)
+{
+    int bits = 0;
+    uint64_t sum_dims = 0, max_dim = 0;
+    for (uint64_t d : j.dims) {
+        bits += d > 1 ? 64 - __builtin_clzll(d - 1) : 0;
+        sum_dims += d;
+        max_dim = std::max(max_dim, d);
+    }
+    const uint64_t W = bits <= 64 ? 8 : 16;
+    const uint64_t sv = j.precision == ALTO_PREC_F64 ? 8 : 4;
+    uint64_t b = j.nnz * (W + 8);                           // ALTO keys + values
+    b += (uint64_t)j.nmodes * j.nnz * (W + sv);             // per-mode streams
+    b += j.nnz * (W + 8 + 8);                               // construction temporaries (sort buffers)
+    b += sum_dims * j.rank * 8 * 2 + 2 * max_dim * j.rank * 8;  // factors, copies, M, scratch
+    b += 64ull << 20;                                        // LUTs, Grams, partials
+    return b;
+}
+
+}  // namespace
+
+struct alto_server {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Job*> queue;                  // waiting for admission, arrival order
+    std::map<uint64_t, std::unique_ptr<Job>> jobs;
+    uint64_t budget = 0, in_use = 0, peak = 0, next_id = 1;
+    int slots = 1, running = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::mutex rand_mu;  // glibc srand/rand is process-global
+    int device = 0;
+
+    Job* admit_locked()
+    {
+        if (running >= slots) return nullptr;
+        for (auto it = queue.begin(); it != queue.end(); ++it) {
+            Job* j = *it;
+            if (in_use + j->footprint <= budget || (running == 0 && in_use == 0)) {
+                queue.erase(it);
+                in_use += j->footprint;
+                peak = std::max(peak, in_use);
+                ++running;
+                j->status = 1;
+                return j;
+            }
+        }
+        return nullptr;
+    }
+
+    void run_job(Job* j, hipStream_t s)
+    {
+        j->t_start = now_s();
+        const bool dev = j->dev_vals != nullptr;
+        std::vector<const uint64_t*> rows(j->nmodes);
+        for (int m = 0; m < j->nmodes; ++m) rows[m] = dev ? j->dev_cidx[m] : j->cidx[m].data();
+        alto_tensor_t* t = nullptr;
+        int st = alto_tensor_create(j->nmodes, j->dims.data(), j->nnz, rows.data(), dev ? j->dev_vals : j->vals.data(),
+                                    dev ? ALTO_PTR_DEVICE : ALTO_PTR_HOST, ALTO_VAL_F64, 0, 1, (alto_stream_t)s, &t);
+        j->U.assign(j->nmodes, {});
+        std::vector<double*> up(j->nmodes);
+        for (int m = 0; m < j->nmodes; ++m) {
+            j->U[m].resize(j->dims[m] * j->rank);
+            up[m] = j->U[m].data();
+        }
+        j->lambda.assign(j->rank, 1.0);
+        if (st == ALTO_OK) {
+            std::lock_guard<std::mutex> g(rand_mu);
+            st = alto_kruskal_random_init(j->nmodes, j->dims.data(), j->rank, j->seed, up.data(), j->lambda.data());
+        }
+        alto_cpd_session_t* sess = nullptr;
+        if (st == ALTO_OK) st = alto_cpd_session_create(t, j->rank, j->precision, up.data(), (alto_stream_t)s, &sess);
+        int ran = 0, conv = 0;
+        if (st == ALTO_OK) st = alto_cpd_session_iterate(sess, j->max_iters, j->epsilon, nullptr, &ran, &conv, 0);
+        if (st == ALTO_OK) st = alto_cpd_session_get(sess, up.data(), j->lambda.data(), &j->fit, nullptr);
+        if (st != ALTO_OK) j->error = alto_last_error();
+        if (sess) alto_cpd_session_destroy(sess);
+        if (t) alto_tensor_destroy(t);
+        j->iters = ran;
+        // the inputs are no longer needed
+        std::vector<std::vector<uint64_t>>().swap(j->cidx);
+        std::vector<double>().swap(j->vals);
+        j->t_end = now_s();
+        j->status = st == ALTO_OK ? 2 : st;
+    }
+
+    void worker_main()
+    {
+        (void)hipSetDevice(device);
+        hipStream_t s = nullptr;
+        (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        for (;;) {
+            Job* j = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || (j = admit_locked()) != nullptr; });
+                if (!j) break;
+            }
+            run_job(j, s);
+            {
+                std::lock_guard<std::mutex> g(mu);
+                in_use -= j->footprint;
+                --running;
+            }
+            cv_work.notify_all();
+            cv_done.notify_all();
+        }
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+extern "C" {
+
+int alto_server_create(uint64_t hbm_budget_bytes, int max_concurrent, alto_server_t** out)
+{
+    if (!out || max_concurrent < 1 || hbm_budget_bytes == 0) return ALTO_ERR_ARG;
+    alto_server* S = new alto_server();
+    S->budget = hbm_budget_bytes;
+    S->slots = max_concurrent;
+    (void)hipGetDevice(&S->device);
+    for (int k = 0; k < max_concurrent; ++k) S->workers.emplace_back([S] { S->worker_main(); });
+    *out = S;
+    return ALTO_OK;
+}
+
+int alto_server_submit(alto_server_t* S, const alto_job_desc_t* d, uint64_t* job_id)
+{
+    if (!S || !d || !job_id || d->nmodes < 1 || d->nmodes > ALTO_MAX_MODES || !d->dims || !d->cidx || !d->vals ||
+        d->rank < 1 || d->rank > 128 || d->max_iters < 1 || d->nnz < 1)
+        return ALTO_ERR_ARG;
+    auto j = std::make_unique<Job>();
+    j->nmodes = d->nmodes;
+    j->dims.assign(d->dims, d->dims + d->nmodes);
+    j->nnz = d->nnz;
+    if (d->ptr_kind == ALTO_PTR_DEVICE) {
+        j->dev_cidx.assign(d->cidx, d->cidx + d->nmodes);
+        j->dev_vals = d->vals;
+    } else {
+        j->cidx.resize(d->nmodes);
+        for (int m = 0; m < d->nmodes; ++m) j->cidx[m].assign(d->cidx[m], d->cidx[m] + d->nnz);
+        j->vals.assign(d->vals, d->vals + d->nnz);
+    }
+    j->rank = d->rank;
+    j->max_iters = d->max_iters;
+    j->epsilon = d->epsilon;
+    j->seed = d->seed;
+    j->precision = d->precision;
+    j->footprint = footprint_of(*j);
+    j->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> g(S->mu);
+        j->id = S->next_id++;
+        *job_id = j->id;
+        S->queue.push_back(j.get());
+        S->jobs[j->id] = std::move(j);
+    }
+    S->cv_work.notify_all();
+    return ALTO_OK;
+}
+
+int alto_server_result(alto_server_t* S, uint64_t job_id, int wait, alto_job_result_t* res, double* const* U,
+                       double* lambda)
+{
+    if (!S || !res) return ALTO_ERR_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end()) return ALTO_ERR_ARG;
+    Job* j = it->second.get();
+    if (wait) S->cv_done.wait(lk, [&] { return j->status == 2 || j->status < 0; });
+    res->status = j->status;
+    res->iters = j->iters;
+    res->fit = j->fit;
+    res->t_submit = j->t_submit;
+    res->t_start = j->t_start;
+    res->t_end = j->t_end;
+    res->footprint = j->footprint;
+    if (j->status == 2) {
+        if (U)
+            for (int m = 0; m < j->nmodes; ++m)
+                if (U[m]) std::copy(j->U[m].begin(), j->U[m].end(), U[m]);
+        if (lambda) std::copy(j->lambda.begin(), j->lambda.end(), lambda);
+    }
+    return ALTO_OK;
+}
+
+int alto_server_release(alto_server_t* S, uint64_t job_id)
+{
+    if (!S) return ALTO_ERR_ARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    auto it = S->jobs.find(job_id);
+    if (it == S->jobs.end() || it->second->status < 2) return ALTO_ERR_BUSY;
+    S->jobs.erase(it);
+    return ALTO_OK;
+}
+
+int alto_server_stats(alto_server_t* S, alto_server_stats_t* st)
+{
+    if (!S || !st) return ALTO_ERR_ARG;
+    std::lock_guard<std::mutex> g(S->mu);
+    st->queued = (int)S->queue.size();
+    st->running = S->running;
+    st->hbm_in_use = S->in_use;
+    st->hbm_peak = S->peak;
+    st->hbm_budget = S->budget;
+    return ALTO_OK;
+}
+
+int alto_server_destroy(alto_server_t* S)
+{
+    if (!S) return ALTO_OK;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_done.wait(lk, [&] { return S->queue.empty() && S->running == 0; });
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto& w : S->workers) w.join();
+    delete S;
+    return ALTO_OK;
+}
+
+}  // extern "C"

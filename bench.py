@@ -15,6 +15,8 @@ from __future__ import annotations
 
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -41,13 +43,14 @@ def main():
     ap.add_argument("--nnz", type=int, default=NNZ)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP32/BF16 runs")
+    ap.add_argument("--serve-jobs", type=int, default=100,
+                    help="jobs of the Poisson(50/s) serving stream to run (0 = skip; the full stream is 2000)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the multi-GPU "
               "path is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import alto_amd as A
@@ -116,6 +119,10 @@ def main():
                               "fit": float(vfits[-1]), "fit_delta_vs_fp64": float(abs(vfits[-1] - fits[-1]))}
             log(f"{name} CP-ALS: {vdt / args.steps * 1e3:.3f} ms/iter, fit {vfits[-1]:.6f}")
 
+    serving = None
+    if args.serve_jobs > 0:
+        serving = serve_leg(args.serve_jobs)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(T, U0)
@@ -143,9 +150,49 @@ def main():
         "alto_create_s": create_s,
         "cpd_timers_s": tm,
         "precision_variants": variants,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def serve_leg(n_jobs):
+    """The job stream (Poisson 50/s, 3-5 modes, dims log-uniform [64, 2^20],
+    nnz log-uniform [1e4, 2e7], Zipf(1.1), rank in {8,16,32,64}, 20 iterations
+    or fit change < 1e-5) under a 200 GB HBM budget: the plain FP64 per-job path
+    (one job at a time) against the batched server (8 concurrent jobs) in FP32
+    and BF16."""
+    import torch
+
+    from alto_amd import PREC_BF16, PREC_F32, PREC_F64
+    from alto_amd.serve import job_stream, materialize, run_stream
+
+    t0 = time.time()
+    jobs = materialize(job_stream(n_jobs, seed=5))
+    torch.cuda.synchronize()
+    log(f"serving: generated {n_jobs} jobs ({sum(j['nnz'] for j in jobs) / 1e6:.0f}M nonzeros) in {time.time() - t0:.1f}s")
+    out = {"jobs": n_jobs, "arrival_rate_per_s": 50.0, "hbm_budget_gb": 200}
+    for name, prec, slots in (("fp64_per_job", PREC_F64, 1), ("fp32_batched", PREC_F32, 8),
+                              ("bf16_batched", PREC_BF16, 8)):
+        r = run_stream(jobs, prec, slots)
+        fits = r.pop("fits")
+        out[name] = r
+        log(f"serving {name}: {r['jobs_per_s']:.1f} jobs/s p50 {r['p50_latency_s'] * 1e3:.1f} ms "
+            f"p99 {r['p99_latency_s'] * 1e3:.1f} ms")
+        out[name + "_mean_fit"] = float(np.mean(fits))
+    for name in ("fp32_batched", "bf16_batched"):
+        out[name + "_speedup_vs_fp64"] = out[name]["jobs_per_s"] / out["fp64_per_job"]["jobs_per_s"]
+    # saturation capacity: every job available at t = 0
+    for name, prec, slots in (("fp64_per_job_saturated", PREC_F64, 1), ("bf16_batched_saturated", PREC_BF16, 8)):
+        r = run_stream(jobs, prec, slots, time_scale=0.0)
+        r.pop("fits")
+        out[name] = r
+        log(f"serving {name}: {r['jobs_per_s']:.1f} jobs/s")
+    out["bf16_saturated_speedup_vs_fp64"] = (out["bf16_batched_saturated"]["jobs_per_s"] /
+                                             out["fp64_per_job_saturated"]["jobs_per_s"])
+    del jobs
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(T, U0):

@@ -192,12 +192,61 @@ int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* fac
 int alto_kruskal_random_init(int nmodes, const uint64_t* dims, uint64_t rank, unsigned seed,
                              double* const* U, double* lambda);
 
+/* ---- serving: continuous batching of CP-ALS jobs ----------------------------
+ * No reference counterpart (the reference runs one decomposition per
+ * process).  A server owns max_concurrent workers, each with its own
+ * non-blocking HIP stream.  Submitted jobs wait in arrival order and are
+ * admitted as soon as a worker is free and their estimated HBM footprint fits
+ * the budget (a smaller later job may pass a head job that does not fit);
+ * admitted jobs build their ALTO tensor and run CP-ALS concurrently on the
+ * device.  Inputs are copied at submit, so the caller may free them. */
+typedef struct alto_server alto_server_t;
+typedef struct {
+    int nmodes;
+    const uint64_t* dims;
+    uint64_t nnz;
+    const uint64_t* const* cidx; /* nmodes pointers, zero-based coordinates */
+    const double* vals;
+    int ptr_kind;    /* ALTO_PTR_HOST: copied at submit; ALTO_PTR_DEVICE: used in place,
+                        the caller keeps them alive until the job finished */
+    uint64_t rank;
+    int max_iters;
+    double epsilon;  /* stop when |fit delta| < epsilon (after the first iteration) */
+    unsigned seed;   /* KruskalModelRandomInit seed */
+    int precision;   /* ALTO_PREC_* */
+} alto_job_desc_t;
+typedef struct {
+    int status; /* 0 queued, 1 running, 2 done, < 0 error */
+    int iters;
+    double fit;
+    double t_submit, t_start, t_end; /* seconds on the server's monotonic clock */
+    uint64_t footprint;              /* estimated HBM bytes while running */
+} alto_job_result_t;
+typedef struct {
+    int queued, running;
+    uint64_t hbm_in_use, hbm_peak, hbm_budget;
+} alto_server_stats_t;
+int alto_server_create(uint64_t hbm_budget_bytes, int max_concurrent, alto_server_t** out);
+int alto_server_submit(alto_server_t* s, const alto_job_desc_t* job, uint64_t* job_id);
+/* wait != 0 blocks until the job finished; U (nmodes host pointers) and
+ * lambda may be NULL. */
+int alto_server_result(alto_server_t* s, uint64_t job_id, int wait, alto_job_result_t* res,
+                       double* const* U, double* lambda);
+int alto_server_release(alto_server_t* s, uint64_t job_id);
+int alto_server_stats(alto_server_t* s, alto_server_stats_t* stats);
+/* waits for every submitted job, then stops the workers */
+int alto_server_destroy(alto_server_t* s);
+
 /* ---- device memory helpers (for callers without their own allocator) ---- */
 int alto_device_malloc(void** p, uint64_t bytes);
 int alto_device_free(void* p);
 int alto_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 int alto_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 int alto_device_synchronize(void);
+/* Returns the library's cached device blocks to the runtime (the library
+ * reuses freed buffers internally so that concurrent work never blocks on
+ * hipFree). */
+int alto_device_trim(void);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,64 @@
+"""Serving: each job's result is the same alone and in a concurrent mix
+(FP64 fit within 1e-10 relative, FP32 within 1e-6, BF16 within 1e-4), the
+admission loop respects the HBM budget, and a failed job is isolated."""
+import numpy as np
+import pytest
+
+from tests._tensors import zipf_tensor
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(n, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nm = int(rng.integers(3, 6))
+        dims = [int(rng.integers(40, 400)) for _ in range(nm)]
+        nnz = int(rng.integers(2000, 30000))
+        cidx, vals = zipf_tensor(dims, nnz, 100 + k)
+        out.append(dict(dims=dims, cidx=cidx, vals=vals, rank=int(rng.choice([8, 16, 32])), seed=7 + k))
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-6), (2, 1e-4)])
+def test_alone_equals_in_mix(prec, tol):
+    from alto_amd.serve import Server
+    jobs = _jobs(12)
+    alone = []
+    s1 = Server(8 << 30, 1)
+    for j in jobs:
+        jid = s1.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], 15, 1e-5, j["seed"], prec)
+        r, U, lam = s1.result(jid, True, j["dims"], j["rank"])
+        assert r.status == 2
+        alone.append((r.fit, r.iters, U))
+    s1.close()
+    s8 = Server(8 << 30, 8)
+    ids = [s8.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], 15, 1e-5, j["seed"], prec) for j in jobs]
+    for j, jid, (fit, iters, U) in zip(jobs, ids, alone):
+        r, U2, lam2 = s8.result(jid, True, j["dims"], j["rank"])
+        assert r.status == 2
+        assert abs(r.fit - fit) <= tol * max(1.0, abs(fit))
+        if prec == 0:
+            assert r.iters == iters
+            for a, b in zip(U, U2):
+                np.testing.assert_allclose(a, b, rtol=1e-7, atol=1e-9)
+    s8.close()
+
+
+def test_budget_admission_and_error_isolation():
+    from alto_amd.serve import Server
+    jobs = _jobs(6, seed=1)
+    s = Server(200 << 20, 4)  # tiny budget: jobs are admitted one at a time
+    ids = [s.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], 5, 0.0, j["seed"]) for j in jobs]
+    # a job whose coordinates exceed its dims fails alone
+    bad = jobs[0]
+    bad_c = [c.copy() for c in bad["cidx"]]
+    bad_c[0][0] = bad["dims"][0] + 5
+    bid = s.submit(bad["dims"], bad_c, bad["vals"], bad["rank"], 5, 0.0, 1)
+    for jid in ids:
+        assert s.result(jid)[0].status == 2
+    assert s.result(bid)[0].status < 0
+    st = s.stats()
+    assert st.hbm_peak <= max(st.hbm_budget, max(s.result(i)[0].footprint for i in ids))
+    s.close()
