@@ -35,6 +35,8 @@ void set_error_hip(hipError_t e, const char* what);
 
 // ---------------------------------------------------------------- device memory
 // Thin owning wrapper; all device allocations of the library go through it.
+// Blocks go back to a cache shared by all streams on release, so a buffer
+// must be released only after the work that uses it has completed.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;

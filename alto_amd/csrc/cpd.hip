@@ -117,7 +117,8 @@ static int cpd_alloc(alto_tensor* t, int R, CpdState<UT, ST, MT>& c)
 static bool gram_f64(int prec) { return prec == ALTO_PREC_F64; }
 
 template <typename UT, typename ST, typename MT>
-static int cpd_upload(alto_tensor* t, CpdState<UT, ST, MT>& c, double* const* Uh, hipStream_t s)
+static int cpd_upload(alto_tensor* t, CpdState<UT, ST, MT>& c, double* const* Uh, hipStream_t s,
+                      uint64_t dev_seed = 0)
 {
     const int R = c.R;
     std::vector<UT> tmp;
@@ -126,7 +127,18 @@ static int cpd_upload(alto_tensor* t, CpdState<UT, ST, MT>& c, double* const* Uh
     ALTO_TRY(fill_value<double>(ones.as<double>(), R, 1.0, s));
     for (int m = 0; m < c.N; ++m) {
         uint64_t n = t->dims[m] * R;
-        if (std::is_same<UT, double>::value) {
+        if (!Uh) {  // seeded on the device (counter-based), then converted
+            DevBuf d64;
+            double* dst = std::is_same<UT, double>::value ? reinterpret_cast<double*>(c.U[m].p) : nullptr;
+            if (!dst) {
+                ALTO_TRY(d64.alloc(n * 8));
+                dst = d64.as<double>();
+            }
+            ALTO_TRY(random_factor_device(dst, n, dev_seed, m, s));
+            if (!std::is_same<UT, double>::value)
+                ALTO_TRY((scale_device<double, UT>(dst, t->dims[m], R, ones.as<double>(), c.u(m), s)));
+            HIP_TRY(hipStreamSynchronize(s));
+        } else if (std::is_same<UT, double>::value) {
             HIP_TRY(hipMemcpyAsync(c.U[m].p, Uh[m], n * 8, hipMemcpyHostToDevice, s));
         } else {
             tmp.resize(n);
@@ -224,7 +236,7 @@ struct CpdSessionBase {
 template <typename UT, typename ST, typename MT>
 struct CpdSession : CpdSessionBase {
     CpdState<UT, ST, MT> c;
-    int init(alto_tensor* tt, int rank, int precision, double* const* Uh, hipStream_t st)
+    int init(alto_tensor* tt, int rank, int precision, double* const* Uh, hipStream_t st, uint64_t dev_seed = 0)
     {
         t = tt;
         R = rank;
@@ -233,7 +245,7 @@ struct CpdSession : CpdSessionBase {
         c.prec = prec;
         ALTO_TRY(prepare_mode_streams(t, prec == ALTO_PREC_F64, prec != ALTO_PREC_F64, s));
         ALTO_TRY(cpd_alloc(t, R, c));
-        ALTO_TRY(cpd_upload(t, c, Uh, s));
+        ALTO_TRY(cpd_upload(t, c, Uh, s, dev_seed));
         ALTO_TRY(fill_value<double>(c.lambda.template as<double>(), R, 1.0, s));
         ALTO_TRY(normsq_device(t->vals.as<double>(), t->nnz, c.nxpart.template as<double>(), c.nx.template as<double>(), s));
         HIP_TRY(hipMemcpyAsync(&normX, c.nx.p, 8, hipMemcpyDeviceToHost, s));
@@ -281,7 +293,8 @@ struct CpdSession : CpdSessionBase {
     }
 };
 
-static int session_new(alto_tensor* t, int R, int prec, double* const* Uh, hipStream_t s, CpdSessionBase** out)
+static int session_new(alto_tensor* t, int R, int prec, double* const* Uh, hipStream_t s, CpdSessionBase** out,
+                       uint64_t dev_seed = 0)
 {
     if (R < 1 || R > 128) {
         set_error("cpd: rank must be 1..128");
@@ -292,19 +305,19 @@ static int session_new(alto_tensor* t, int R, int prec, double* const* Uh, hipSt
     switch (prec) {
     case ALTO_PREC_F64: {
         auto* x = new CpdSession<double, double, double>();
-        st = x->init(t, R, prec, Uh, s);
+        st = x->init(t, R, prec, Uh, s, dev_seed);
         b = x;
         break;
     }
     case ALTO_PREC_F32: {
         auto* x = new CpdSession<float, float, float>();
-        st = x->init(t, R, prec, Uh, s);
+        st = x->init(t, R, prec, Uh, s, dev_seed);
         b = x;
         break;
     }
     case ALTO_PREC_BF16: {
         auto* x = new CpdSession<float, bf16_t, float>();
-        st = x->init(t, R, prec, Uh, s);
+        st = x->init(t, R, prec, Uh, s, dev_seed);
         b = x;
         break;
     }
@@ -414,6 +427,19 @@ int alto_cpd_session_create(alto_tensor_t* t, uint64_t rank, int precision, doub
     }
     CpdSessionBase* b = nullptr;
     ALTO_TRY(session_new(t, (int)rank, precision, U, (hipStream_t)stream, &b));
+    *out = reinterpret_cast<alto_cpd_session_t*>(b);
+    return ALTO_OK;
+}
+
+int alto_cpd_session_create_seeded(alto_tensor_t* t, uint64_t rank, int precision, uint64_t seed,
+                                   alto_stream_t stream, alto_cpd_session_t** out)
+{
+    if (!t || !out) {
+        set_error("alto_cpd_session_create_seeded: null argument");
+        return ALTO_ERR_ARG;
+    }
+    CpdSessionBase* b = nullptr;
+    ALTO_TRY(session_new(t, (int)rank, precision, nullptr, (hipStream_t)stream, &b, seed));
     *out = reinterpret_cast<alto_cpd_session_t*>(b);
     return ALTO_OK;
 }

@@ -875,10 +875,8 @@ static int create_impl(int nmode, const uint64_t* dims, uint64_t nnz, const uint
     hipLaunchKernelGGL(k_gather_f64, dim3(grid_for(nnz)), dim3(256), 0, s, vtmp.as<double>(), pay.as<uint32_t>(), nnz,
                        t->vals.as<double>());
     HIP_TRY(hipGetLastError());
-    for (int m = 0; m < nmode; ++m) dcoord[m].release();
-    dvals.release();
-    vtmp.release();
-    pay.release();
+    // the staging buffers are released only after the stream has drained
+    // (end of this function): the allocator hands freed blocks to other streams
 
     // oidx flag (include/alto.hpp:250-258)
     for (int m = 0; m < nmode; ++m)

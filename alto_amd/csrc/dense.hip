@@ -190,6 +190,17 @@ __global__ void __launch_bounds__(256) k_pinv(const double* __restrict__ grams, 
         Q[k] = i == c ? 1.0 : 0.0;
     }
     __syncthreads();
+    // a non-finite V has no pseudo-inverse: return NaN at once
+    __shared__ int s_bad;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int k = tid; k < R * R; k += blockDim.x)
+        if (!isfinite(A[k])) s_bad = 1;
+    __syncthreads();
+    if (s_bad) {
+        for (int k = tid; k < R * R; k += blockDim.x) Pout[k] = NAN;
+        return;
+    }
     for (int sweep = 0; sweep < 60; ++sweep) {
         double off = 0, tot = 0;
         for (int k = tid; k < R * R; k += blockDim.x) {
@@ -472,6 +483,28 @@ __global__ void k_sumsq_partial(const double* __restrict__ v, uint64_t n, double
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
+// Counter-based uniform (0, 1] factor entries: SplitMix64 of (seed, mode,
+// index).  Used for models seeded on the device (served jobs).
+__global__ void k_random_factor(double* __restrict__ U, uint64_t n, uint64_t seed, int mode)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed * 0x9E3779B97F4A7C15ull + ((uint64_t)mode << 56) + i + 1;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        U[i] = ((double)(z >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+    }
+}
+
+int random_factor_device(double* U, uint64_t n, uint64_t seed, int mode, hipStream_t s)
+{
+    uint64_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    hipLaunchKernelGGL(k_random_factor, dim3((unsigned)(b ? b : 1)), dim3(256), 0, s, U, n, seed, mode);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
 // ------------------------------------------------------------------ host wrappers
 int solve_blocks(uint64_t I, int R)
 {
@@ -490,7 +523,7 @@ int chol_device(const double* grams, int nmode, int mode, int R, double* L, doub
         return ALTO_ERR_UNSUPPORTED;
     }
     if (R <= 64)
-        hipLaunchKernelGGL((k_chol<true>), dim3(1), dim3(256), (size_t)R * R * 8, s, grams, nmode, mode, R, wsA, L, status);
+        hipLaunchKernelGGL((k_chol<true>), dim3(1), dim3(64), (size_t)R * R * 8, s, grams, nmode, mode, R, wsA, L, status);
     else
         hipLaunchKernelGGL((k_chol<false>), dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, L, status);
     hipLaunchKernelGGL(k_pinv, dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, wsQ, P, status);
@@ -534,6 +567,7 @@ int scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, hipStrea
 template int scale_device<double, double>(double*, uint64_t, int, const double*, double*, hipStream_t);
 template int scale_device<float, float>(float*, uint64_t, int, const double*, float*, hipStream_t);
 template int scale_device<float, bf16_t>(float*, uint64_t, int, const double*, bf16_t*, hipStream_t);
+template int scale_device<double, float>(double*, uint64_t, int, const double*, float*, hipStream_t);
 
 constexpr int INNER_BLOCKS = 64;
 

@@ -22,6 +22,7 @@ template <typename UT, typename MT>
 int inner_partial_device(const UT* U, const MT* Mm, uint64_t I, int R, double* part, hipStream_t s);
 int fit_device(const double* grams, int nmode, int R, const double* lambda, const double* ipart, double normX,
                double* out, hipStream_t s);
+int random_factor_device(double* U, uint64_t n, uint64_t seed, int mode, hipStream_t s);
 int normsq_device(const double* v, uint64_t n, double* part, double* out, hipStream_t s);
 
 }  // namespace alto

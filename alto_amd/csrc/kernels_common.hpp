@@ -68,13 +68,13 @@ __device__ inline T group_sum(T v)
 __device__ inline void atomic_add_hw(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ inline void atomic_add_hw(float* p, float v) { unsafeAtomicAdd(p, v); }
 
-template <typename AT, typename OT, int G, int E>
+template <typename AT, typename OT, int G, int E, bool CONTIG = false>
 __device__ inline void flush_row(OT* out, uint32_t row, int R, int gl, const AT (&acc)[E])
 {
     OT* o = out + (uint64_t)row * R;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-        const int c = gl + q * G;
+        const int c = CONTIG ? gl * E + q : gl + q * G;
         if (c < R) atomic_add_hw(&o[c], (OT)acc[q]);
     }
 }

@@ -30,6 +30,7 @@ int tuned_seg(int R)
 }
 static int g_chunk = 0;
 static int g_nt = 0;
+static int g_variant = 0;
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
 {
@@ -44,14 +45,82 @@ int set_seg(int v)
     return ALTO_OK;
 }
 
-// Columns are strided over the group: lane gl holds c = gl + q*G, so each
-// gather and each atomic flush instruction touches G contiguous elements.
-template <typename FT, typename AT, int G, int E>
+// Column mapping of a lane group: strided (lane gl holds c = gl + q*G, so
+// each gather / atomic instruction touches G contiguous elements) or
+// contiguous (lane gl holds c = gl*E + q, loaded with one 16-byte load where
+// the row allows it).
+template <bool CONTIG, int G, int E>
+__device__ inline int col_of(int gl, int q) { return CONTIG ? gl * E + q : gl + q * G; }
+
+template <typename FT, int E> struct VecLoad {
+    static constexpr bool ok = false;
+};
+template <> struct VecLoad<double, 2> {
+    static constexpr bool ok = true;
+    template <typename AT> __device__ static void load(const double* p, AT (&x)[2])
+    {
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        x[0] = (AT)a.x;
+        x[1] = (AT)a.y;
+    }
+};
+template <> struct VecLoad<double, 4> {
+    static constexpr bool ok = true;
+    template <typename AT> __device__ static void load(const double* p, AT (&x)[4])
+    {
+        const double2 a = *reinterpret_cast<const double2*>(p);
+        const double2 b = *reinterpret_cast<const double2*>(p + 2);
+        x[0] = (AT)a.x; x[1] = (AT)a.y; x[2] = (AT)b.x; x[3] = (AT)b.y;
+    }
+};
+template <> struct VecLoad<float, 4> {
+    static constexpr bool ok = true;
+    template <typename AT> __device__ static void load(const float* p, AT (&x)[4])
+    {
+        const float4 a = *reinterpret_cast<const float4*>(p);
+        x[0] = (AT)a.x; x[1] = (AT)a.y; x[2] = (AT)a.z; x[3] = (AT)a.w;
+    }
+};
+template <> struct VecLoad<float, 2> {
+    static constexpr bool ok = true;
+    template <typename AT> __device__ static void load(const float* p, AT (&x)[2])
+    {
+        const float2 a = *reinterpret_cast<const float2*>(p);
+        x[0] = (AT)a.x; x[1] = (AT)a.y;
+    }
+};
+template <> struct VecLoad<bf16_t, 4> {
+    static constexpr bool ok = true;
+    template <typename AT> __device__ static void load(const bf16_t* p, AT (&x)[4])
+    {
+        const uint2 a = *reinterpret_cast<const uint2*>(p);
+        x[0] = (AT)__uint_as_float(a.x << 16); x[1] = (AT)__uint_as_float(a.x & 0xffff0000u);
+        x[2] = (AT)__uint_as_float(a.y << 16); x[3] = (AT)__uint_as_float(a.y & 0xffff0000u);
+    }
+};
+template <> struct VecLoad<bf16_t, 2> {
+    static constexpr bool ok = true;
+    template <typename AT> __device__ static void load(const bf16_t* p, AT (&x)[2])
+    {
+        const uint32_t a = *reinterpret_cast<const uint32_t*>(p);
+        x[0] = (AT)__uint_as_float(a << 16); x[1] = (AT)__uint_as_float(a & 0xffff0000u);
+    }
+};
+
+template <typename FT, typename AT, int G, int E, bool CONTIG = false>
 __device__ inline void load_cols(const FT* __restrict__ row, int gl, int R, AT (&x)[E])
 {
+    if constexpr (CONTIG && VecLoad<FT, E>::ok) {
+        const int c0 = gl * E;
+        // bf16 rows of odd rank are only 2-byte aligned: element loads there
+        if (c0 + E <= R && (sizeof(FT) >= 4 || (R & 1) == 0)) {
+            VecLoad<FT, E>::load(row + c0, x);
+            return;
+        }
+    }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-        const int c = gl + q * G;
+        const int c = col_of<CONTIG, G, E>(gl, q);
         x[q] = c < R ? to_acc<AT>(row[c]) : (AT)0;
     }
 }
@@ -63,7 +132,7 @@ __device__ inline T stream_ld(const T* p)
     else return *p;
 }
 
-template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J, bool NT>
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J, bool NT, bool CONTIG>
 __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
 {
     const int lane = threadIdx.x & 63;
@@ -124,7 +193,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
                         const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
                         const uint32_t off = (uint32_t)__shfl((int)offs[o], j, G);
                         AT x[E];
-                        load_cols<FT, AT, G, E>(U + off, gl, R, x);
+                        load_cols<FT, AT, G, E, CONTIG>(U + off, gl, R, x);
 #pragma unroll
                         for (int q = 0; q < E; ++q) prod[jj][q] *= x[q];
                     }
@@ -135,7 +204,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
             for (int jj = 0; jj < J; ++jj) {
                 if (j0 + jj < cnt) {
                     if (rj[jj] != cur) {
-                        if (cur != 0xffffffffu) flush_row<AT, OT, G, E>(a.out, cur, R, gl, acc);
+                        if (cur != 0xffffffffu) flush_row<AT, OT, G, E, CONTIG>(a.out, cur, R, gl, acc);
                         cur = rj[jj];
 #pragma unroll
                         for (int q = 0; q < E; ++q) acc[q] = 0;
@@ -146,7 +215,7 @@ __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
             }
         }
     }
-    if (cur != 0xffffffffu) flush_row<AT, OT, G, E>(a.out, cur, R, gl, acc);
+    if (cur != 0xffffffffu) flush_row<AT, OT, G, E, CONTIG>(a.out, cur, R, gl, acc);
 }
 
 // Phi: per nonzero pi = prod_{m != n} U_m[row_m,:], d = <B[row,:], pi>,
@@ -249,11 +318,41 @@ static int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const*
     set_factors(t, mode, U, a);
     a.val = val;
     a.out = out;
-    dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
-    RANK_DISPATCH(R, t->pack.pw, {
-        if (g_nt) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, G, E, PW, 4, true>), grid, dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, G, E, PW, 4, false>), grid, dim3(256), 0, s, a);
-    });
+    // variants: 0 = 2 strided columns per lane, 4 nonzeros in flight per group;
+    // 1 = same with 8 in flight; 2 = 4 contiguous columns per lane (16-byte
+    // loads), 4 in flight; 3 = 4 contiguous columns, 8 in flight
+    const int v = g_variant;
+    const int E = v >= 2 ? 4 : 2;
+    int G = 4;
+    while (G * E < R && G < 64) G <<= 1;
+    dim3 grid = stream_grid(t->nnz, seg, G, a);
+#define L_(GG, EE, JJ, CC)                                                                                      \
+    do {                                                                                                         \
+        if (t->pack.pw == 1) {                                                                                   \
+            if (g_nt) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, true, CC>), grid, dim3(256), 0, s, a); \
+            else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, false, CC>), grid, dim3(256), 0, s, a);     \
+        } else {                                                                                                 \
+            hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 2, JJ, false, CC>), grid, dim3(256), 0, s, a);          \
+        }                                                                                                        \
+    } while (0)
+#define LG_(EE, JJ, CC)                                                                                          \
+    do {                                                                                                         \
+        switch (G) {                                                                                             \
+        case 4: L_(4, EE, JJ, CC); break;                                                                        \
+        case 8: L_(8, EE, JJ, CC); break;                                                                        \
+        case 16: L_(16, EE, JJ, CC); break;                                                                      \
+        case 32: L_(32, EE, JJ, CC); break;                                                                      \
+        default: L_(64, EE, JJ, CC); break;                                                                      \
+        }                                                                                                        \
+    } while (0)
+    switch (v) {
+    case 1: LG_(2, 8, false); break;
+    case 2: LG_(4, 4, true); break;
+    case 3: LG_(4, 8, true); break;
+    default: LG_(2, 4, false); break;
+    }
+#undef LG_
+#undef L_
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }
@@ -328,6 +427,11 @@ int alto_tune_set(int param, int value)
 {
     if (param == ALTO_TUNE_SEG) return set_seg(value);
     if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
+    if (param == ALTO_TUNE_MTTKRP_VARIANT) {
+        if (value < 0 || value > 3) return ALTO_ERR_ARG;
+        g_variant = value;
+        return ALTO_OK;
+    }
     if (param == ALTO_TUNE_NONTEMPORAL) {
         g_nt = value ? 1 : 0;
         return ALTO_OK;

@@ -42,6 +42,7 @@ struct Job {
     double epsilon = 0;
     unsigned seed = 0;
     int precision = 0;
+    int want_factors = 0;
     uint64_t footprint = 0;
     // results
     int status = 0;  // 0 queued, 1 running, 2 done, < 0 error code
@@ -86,7 +87,6 @@ struct alto_server {
     int slots = 1, running = 0;
     bool stop = false;
     std::vector<std::thread> workers;
-    std::mutex rand_mu;  // glibc srand/rand is process-global
     int device = 0;
 
     Job* admit_locked()
@@ -115,22 +115,22 @@ struct alto_server {
         alto_tensor_t* t = nullptr;
         int st = alto_tensor_create(j->nmodes, j->dims.data(), j->nnz, rows.data(), dev ? j->dev_vals : j->vals.data(),
                                     dev ? ALTO_PTR_DEVICE : ALTO_PTR_HOST, ALTO_VAL_F64, 0, 1, (alto_stream_t)s, &t);
-        j->U.assign(j->nmodes, {});
-        std::vector<double*> up(j->nmodes);
-        for (int m = 0; m < j->nmodes; ++m) {
-            j->U[m].resize(j->dims[m] * j->rank);
-            up[m] = j->U[m].data();
-        }
-        j->lambda.assign(j->rank, 1.0);
-        if (st == ALTO_OK) {
-            std::lock_guard<std::mutex> g(rand_mu);
-            st = alto_kruskal_random_init(j->nmodes, j->dims.data(), j->rank, j->seed, up.data(), j->lambda.data());
-        }
         alto_cpd_session_t* sess = nullptr;
-        if (st == ALTO_OK) st = alto_cpd_session_create(t, j->rank, j->precision, up.data(), (alto_stream_t)s, &sess);
+        if (st == ALTO_OK) st = alto_cpd_session_create_seeded(t, j->rank, j->precision, j->seed, (alto_stream_t)s, &sess);
         int ran = 0, conv = 0;
         if (st == ALTO_OK) st = alto_cpd_session_iterate(sess, j->max_iters, j->epsilon, nullptr, &ran, &conv, 0);
-        if (st == ALTO_OK) st = alto_cpd_session_get(sess, up.data(), j->lambda.data(), &j->fit, nullptr);
+        if (st == ALTO_OK && j->want_factors) {
+            j->U.assign(j->nmodes, {});
+            std::vector<double*> up(j->nmodes);
+            for (int m = 0; m < j->nmodes; ++m) {
+                j->U[m].resize(j->dims[m] * j->rank);
+                up[m] = j->U[m].data();
+            }
+            j->lambda.assign(j->rank, 0.0);
+            st = alto_cpd_session_get(sess, up.data(), j->lambda.data(), &j->fit, nullptr);
+        } else if (st == ALTO_OK) {
+            st = alto_cpd_session_get(sess, nullptr, nullptr, &j->fit, nullptr);
+        }
         if (st != ALTO_OK) j->error = alto_last_error();
         if (sess) alto_cpd_session_destroy(sess);
         if (t) alto_tensor_destroy(t);
@@ -203,6 +203,7 @@ int alto_server_submit(alto_server_t* S, const alto_job_desc_t* d, uint64_t* job
     j->epsilon = d->epsilon;
     j->seed = d->seed;
     j->precision = d->precision;
+    j->want_factors = d->want_factors;
     j->footprint = footprint_of(*j);
     j->t_submit = now_s();
     {
@@ -232,7 +233,7 @@ int alto_server_result(alto_server_t* S, uint64_t job_id, int wait, alto_job_res
     res->t_start = j->t_start;
     res->t_end = j->t_end;
     res->footprint = j->footprint;
-    if (j->status == 2) {
+    if (j->status == 2 && j->want_factors) {
         if (U)
             for (int m = 0; m < j->nmodes; ++m)
                 if (U[m]) std::copy(j->U[m].begin(), j->U[m].end(), U[m]);

@@ -17,7 +17,7 @@ from . import PREC_F64, PTR_DEVICE, PTR_HOST, _ptr_array, check, lib
 class JobDesc(C.Structure):
     _fields_ = [("nmodes", C.c_int), ("dims", C.c_void_p), ("nnz", C.c_uint64), ("cidx", C.c_void_p),
                 ("vals", C.c_void_p), ("ptr_kind", C.c_int), ("rank", C.c_uint64), ("max_iters", C.c_int),
-                ("epsilon", C.c_double), ("seed", C.c_uint), ("precision", C.c_int)]
+                ("epsilon", C.c_double), ("seed", C.c_uint), ("precision", C.c_int), ("want_factors", C.c_int)]
 
 
 class JobResult(C.Structure):
@@ -59,7 +59,7 @@ class Server:
         self._keep = {}
 
     def submit(self, dims, cidx, vals, rank, max_iters=20, epsilon=1e-5, seed=1, precision=PREC_F64, device=False,
-               nnz=None):
+               nnz=None, want_factors=False):
         """cidx: list of per-mode coordinate arrays (numpy uint64) or device pointers
         (device=True, with nnz given); vals likewise."""
         dims_a = np.ascontiguousarray(dims, dtype=np.uint64)
@@ -76,7 +76,7 @@ class Server:
             n = v.shape[0]
             keep = (dims_a, rows, arrs, v)
         d = JobDesc(len(dims_a), dims_a.ctypes.data, n, C.cast(rows, C.c_void_p), vptr,
-                    PTR_DEVICE if device else PTR_HOST, rank, max_iters, epsilon, seed, precision)
+                    PTR_DEVICE if device else PTR_HOST, rank, max_iters, epsilon, seed, precision, int(want_factors))
         jid = C.c_uint64()
         check(self.L.alto_server_submit(self.h, C.byref(d), C.byref(jid)), "server_submit")
         self._keep[jid.value] = keep

@@ -111,8 +111,10 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * nonzeros walked by one lane group (0 = default).  ALTO_TUNE_CHUNK_LOG2:
  * log2 of the ALTO chunk inside which each mode's stream is row-ordered
  * (0 = default); streams are re-ordered on their next use.
- * ALTO_TUNE_NONTEMPORAL: 1 = stream the nonzeros with non-temporal loads. */
-enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3 };
+ * ALTO_TUNE_NONTEMPORAL: 1 = stream the nonzeros with non-temporal loads.
+ * ALTO_TUNE_MTTKRP_VARIANT: lane mapping of the MTTKRP kernel, 0..3 (2 strided
+ * or 4 contiguous columns per lane; 4 or 8 nonzeros in flight per group). */
+enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4 };
 int alto_tune_set(int param, int value);
 
 /* ---- CP-ALS --------------------------------------------------------------- */
@@ -142,6 +144,10 @@ int alto_cpd_als(alto_tensor_t* t, uint64_t rank, int max_iters, double epsilon,
 typedef struct alto_cpd_session alto_cpd_session_t;
 int alto_cpd_session_create(alto_tensor_t* t, uint64_t rank, int precision, double* const* U,
                             alto_stream_t stream, alto_cpd_session_t** out);
+/* As alto_cpd_session_create, with the initial factors drawn on the device:
+ * U_n[i] = uniform (0, 1] from SplitMix64 of (seed, n, i) (not glibc rand). */
+int alto_cpd_session_create_seeded(alto_tensor_t* t, uint64_t rank, int precision, uint64_t seed,
+                                   alto_stream_t stream, alto_cpd_session_t** out);
 int alto_cpd_session_iterate(alto_cpd_session_t* s, int max_iters, double epsilon, double* fit_trace,
                              int* iters, int* converged, int verbose);
 int alto_cpd_session_get(alto_cpd_session_t* s, double* const* U, double* lambda, double* fit,
@@ -212,8 +218,9 @@ typedef struct {
     uint64_t rank;
     int max_iters;
     double epsilon;  /* stop when |fit delta| < epsilon (after the first iteration) */
-    unsigned seed;   /* KruskalModelRandomInit seed */
+    unsigned seed;   /* initial factors: device counter-based generator keyed by seed */
     int precision;   /* ALTO_PREC_* */
+    int want_factors; /* keep the final factors for alto_server_result */
 } alto_job_desc_t;
 typedef struct {
     int status; /* 0 queued, 1 running, 2 done, < 0 error */
