@@ -233,9 +233,55 @@ struct CpdSessionBase {
     virtual int get(double* const* Uh, double* lambda_h) = 0;
 };
 
+int graphs_enabled();
+
 template <typename UT, typename ST, typename MT>
 struct CpdSession : CpdSessionBase {
     CpdState<UT, ST, MT> c;
+    // one CP-ALS iteration captured as a graph: [0] first iteration (2-norm),
+    // [1] later iterations (max-norm); the fit lands in pinned host memory
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    double* fit_host = nullptr;
+    ~CpdSession() override
+    {
+        for (auto& e : exec)
+            if (e) (void)hipGraphExecDestroy(e);
+        if (fit_host) (void)hipHostFree(fit_host);
+    }
+    int capture(int kind)
+    {
+        if (!fit_host) HIP_TRY(hipHostMalloc((void**)&fit_host, 32, hipHostMallocDefault));
+        PhaseTimer off;
+        ALTO_TRY(off.init(false, s));
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        int st = ALTO_OK;
+        for (int j = 0; j < c.N && st == ALTO_OK; ++j) st = cpd_mode_step(t, c, j, kind, off, s);
+        if (st == ALTO_OK) st = cpd_fit_step(t, c, normX, s);
+        if (st == ALTO_OK) {
+            hipError_t e = hipMemcpyAsync(fit_host, c.fitbuf.p, 24, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) {
+                set_error_hip(e, "hipMemcpyAsync (capture)");
+                st = ALTO_ERR_HIP;
+            }
+        }
+        hipError_t ec = hipStreamEndCapture(s, &g);
+        if (st != ALTO_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (ec != hipSuccess) {
+            set_error_hip(ec, "hipStreamEndCapture");
+            return ALTO_ERR_HIP;
+        }
+        hipError_t ei = hipGraphInstantiate(&exec[kind], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            set_error_hip(ei, "hipGraphInstantiate");
+            return ALTO_ERR_HIP;
+        }
+        return ALTO_OK;
+    }
     int init(alto_tensor* tt, int rank, int precision, double* const* Uh, hipStream_t st, uint64_t dev_seed = 0)
     {
         t = tt;
@@ -254,6 +300,7 @@ struct CpdSession : CpdSessionBase {
     }
     int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) override
     {
+        if (graphs_enabled() && !verbose) return iterate_graph(max_iters, eps, fit_trace, ran);
         PhaseTimer pt;
         ALTO_TRY(pt.init(true, s));
         int done = 0;
@@ -275,6 +322,25 @@ struct CpdSession : CpdSessionBase {
             if (verbose)
                 printf("it: %d\t fit: %g\t fit-delta: %g\ttime(for MTTKRP): %.4f s (%.4f s)\n", it, fit,
                        fabs(prev_fit - fit), now_s() - ti, acc[PH_MTTKRP] - mt_before);
+            if (it > 0 && fabs(prev_fit - fit) < eps) converged = true;
+            prev_fit = fit;
+        }
+        total += now_s() - t0;
+        if (ran) *ran = done;
+        return ALTO_OK;
+    }
+    int iterate_graph(int max_iters, double eps, double* fit_trace, int* ran)
+    {
+        int done = 0;
+        const double t0 = now_s();
+        for (int k = 0; k < max_iters && !converged; ++k, ++it) {
+            const int kind = it == 0 ? 0 : 1;
+            if (!exec[kind]) ALTO_TRY(capture(kind));
+            HIP_TRY(hipGraphLaunch(exec[kind], s));
+            HIP_TRY(hipStreamSynchronize(s));
+            fit = fit_host[0];
+            if (fit_trace) fit_trace[k] = fit;
+            ++done;
             if (it > 0 && fabs(prev_fit - fit) < eps) converged = true;
             prev_fit = fit;
         }

@@ -380,18 +380,29 @@ __global__ void __launch_bounds__(256) k_solve(const MT* __restrict__ M, uint64_
     }
 }
 
-// lambda: iteration 0 -> 2-norm of each column; later -> max(1, max_i U[i,r])
-__global__ void k_lambda(const double* __restrict__ psum, const double* __restrict__ pmax, int nb, int R, int which,
-                         double* __restrict__ lambda)
+// lambda: iteration 0 -> 2-norm of each column; later -> max(1, max_i U[i,r]).
+// One block per column, tree reduction over the per-block partials.
+__global__ void __launch_bounds__(256) k_lambda(const double* __restrict__ psum, const double* __restrict__ pmax, int nb,
+                                                int R, int which, double* __restrict__ lambda)
 {
-    for (int c = threadIdx.x; c < R; c += blockDim.x) {
-        double a = 0, b = -INFINITY;
-        for (int p = 0; p < nb; ++p) {
-            a += psum[(size_t)p * R + c];
-            b = fmax(b, pmax[(size_t)p * R + c]);
-        }
-        lambda[c] = which == 0 ? sqrt(a) : fmax(1.0, fmax(b, 0.0));
+    __shared__ double sa[256], sb[256];
+    const int c = blockIdx.x;
+    double a = 0, b = -INFINITY;
+    for (int p = threadIdx.x; p < nb; p += blockDim.x) {
+        a += psum[(size_t)p * R + c];
+        b = fmax(b, pmax[(size_t)p * R + c]);
     }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) {
+            sa[threadIdx.x] += sa[threadIdx.x + o];
+            sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lambda[c] = which == 0 ? sqrt(sa[0]) : fmax(1.0, fmax(sb[0], 0.0));
 }
 
 // U /= lambda (per column), with the storage copy used by MTTKRP
@@ -549,7 +560,7 @@ template int solve_device<float, float>(const float*, uint64_t, int, const doubl
 
 int lambda_device(const double* psum, const double* pmax, int nb, int R, int which, double* lambda, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lambda, dim3(1), dim3(128), 0, s, psum, pmax, nb, R, which, lambda);
+    hipLaunchKernelGGL(k_lambda, dim3(R), dim3(256), 0, s, psum, pmax, nb, R, which, lambda);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }

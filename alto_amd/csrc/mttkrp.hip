@@ -31,6 +31,8 @@ int tuned_seg(int R)
 static int g_chunk = 0;
 static int g_nt = 0;
 static int g_variant = 0;
+static int g_graphs = 0;
+int graphs_enabled() { return g_graphs; }
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
 {
@@ -427,6 +429,10 @@ int alto_tune_set(int param, int value)
 {
     if (param == ALTO_TUNE_SEG) return set_seg(value);
     if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
+    if (param == ALTO_TUNE_GRAPHS) {
+        g_graphs = value ? 1 : 0;
+        return ALTO_OK;
+    }
     if (param == ALTO_TUNE_MTTKRP_VARIANT) {
         if (value < 0 || value > 3) return ALTO_ERR_ARG;
         g_variant = value;

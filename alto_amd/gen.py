@@ -110,8 +110,10 @@ def poisson_kruskal_counts(dims, rank, expected_nnz, seed, shape=0.5, device="cu
     with columns normalized to sum 1; component r contributes Poisson(lambda_r)
     samples whose mode indices are drawn from U_n[:, r]; duplicates merged into
     counts (the scheme of src/poisson_generator.cpp:200-333).  lambda is scaled
-    so the number of distinct cells is close to expected_nnz."""
+    so that about expected_nnz distinct cells result (the draw count is grown
+    until the distinct count reaches it)."""
     dims = [int(d) for d in dims]
+    N = len(dims)
     gen = torch.Generator(device=device)
     gen.manual_seed(int(seed))
     U = []
@@ -120,32 +122,58 @@ def poisson_kruskal_counts(dims, rank, expected_nnz, seed, shape=0.5, device="cu
         U.append(g / g.sum(0, keepdim=True))
     w = torch.rand(rank, dtype=torch.float64, device=device, generator=gen) + 0.5
     w = w / w.sum()
-    total = float(expected_nnz) * 1.15
-    keys = None
-    draws = torch.poisson(w * total, generator=gen).to(torch.int64)
     cdfs = [torch.cumsum(u, 0) for u in U]
-    parts = []
-    for r in range(rank):
-        m = int(draws[r].item())
-        if m == 0:
-            continue
-        k = torch.zeros(m, dtype=torch.int64, device=device)
-        for n, d in enumerate(dims):
-            u = torch.rand(m, dtype=torch.float64, device=device, generator=gen)
-            idx = torch.searchsorted(cdfs[n][:, r].contiguous(), u * cdfs[n][-1, r]).clamp_(max=d - 1)
-            k = k * d + idx
-        parts.append(k)
-    keys = torch.cat(parts)
-    uk, counts = torch.unique(keys, return_counts=True)
-    if max_nnz is not None and uk.numel() > max_nnz:
-        sel = torch.randperm(uk.numel(), generator=gen, device=device)[:max_nnz]
+    # split the modes into two groups whose index bits each fit 62 bits
+    bits = [max((d - 1).bit_length(), 1) for d in dims]
+    split, acc = 0, 0
+    for n in range(N):
+        if acc + bits[n] > 62:
+            break
+        acc += bits[n]
+        split = n + 1
+
+    def draw(total):
+        m_r = torch.poisson(w * total, generator=gen).to(torch.int64)
+        parts = []
+        for r in range(rank):
+            m = int(m_r[r].item())
+            if m == 0:
+                continue
+            hi = torch.zeros(m, dtype=torch.int64, device=device)
+            lo = torch.zeros(m, dtype=torch.int64, device=device)
+            for n, d in enumerate(dims):
+                u = torch.rand(m, dtype=torch.float64, device=device, generator=gen)
+                idx = torch.searchsorted(cdfs[n][:, r].contiguous(), u * cdfs[n][-1, r]).clamp_(max=d - 1)
+                if n < split:
+                    hi = hi * d + idx
+                else:
+                    lo = lo * d + idx
+            parts.append(torch.stack([hi, lo], 1))
+        return torch.cat(parts, 0)
+
+    total = float(expected_nnz)
+    keys = draw(total)
+    uk, counts = torch.unique(keys, dim=0, return_counts=True)
+    for _ in range(6):
+        if uk.shape[0] >= expected_nnz * 0.98:
+            break
+        extra = draw(total * (expected_nnz - uk.shape[0]) / max(uk.shape[0], 1) + 1024)
+        keys = torch.cat([keys, extra], 0)
+        uk, counts = torch.unique(keys, dim=0, return_counts=True)
+    del keys
+    if max_nnz is not None and uk.shape[0] > max_nnz:
+        sel = torch.randperm(uk.shape[0], generator=gen, device=device)[:max_nnz]
         uk, counts = uk[sel], counts[sel]
-    coords = [None] * len(dims)
-    rem = uk.clone()
-    for n in reversed(range(len(dims))):
+    coords = [None] * N
+    rem = uk[:, 1].clone()
+    for n in reversed(range(split, N)):
         coords[n] = (rem % dims[n]).contiguous()
         rem //= dims[n]
-    return coords, counts.to(torch.float64), U, w * total
+    rem = uk[:, 0].clone()
+    for n in reversed(range(split)):
+        coords[n] = (rem % dims[n]).contiguous()
+        rem //= dims[n]
+    return coords, counts.to(torch.float64), U, w
 
 
 def nnz_bits(dims):

@@ -43,6 +43,14 @@ def main():
     ap.add_argument("--nnz", type=int, default=NNZ)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP32/BF16 runs")
+    ap.add_argument("--apr", action="store_true",
+                    help="also run CP-APR on the 2^20 x 2^18 x 2^16 x 1024 Poisson count tensor (rank 10, 30 x 10)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="also run the 5-mode 50K x 50K x 10K x 2K x 512 Zipf(1.2) FP64 / FP32 / BF16 study (rank 64)")
+    ap.add_argument("--mttkrp-only", type=int, default=0, metavar="REPS",
+                    help="profiling driver: build the tensor, run REPS all-mode MTTKRPs, print nothing else")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="profiling calibration: a 256 MiB device copy (for FETCH_SIZE / WRITE_SIZE scaling)")
     ap.add_argument("--serve-jobs", type=int, default=100,
                     help="jobs of the Poisson(50/s) serving stream to run (0 = skip; the full stream is 2000)")
     args = ap.parse_args()
@@ -59,6 +67,13 @@ def main():
     A.lib()
     stream = torch.cuda.current_stream().cuda_stream
     nnz = args.nnz
+    if args.copy_only:
+        a = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        return
 
     t0 = time.time()
     coords, vals = gen.sparse_coo(DIMS, nnz, SEED, dist="zipf", s=1.1, values="lognormal")
@@ -72,6 +87,15 @@ def main():
     log(f"ALTO creation time:   {create_s:f}")
 
     U0, _ = A.kruskal_random_init(DIMS, RANK, SEED)
+    if args.mttkrp_only:
+        Uf = [torch.from_numpy(u).cuda() for u in U0]
+        outs = [torch.zeros(d, RANK, dtype=torch.float64, device="cuda") for d in DIMS]
+        uptr = A._ptr_array([u.data_ptr() for u in Uf])
+        for _ in range(args.mttkrp_only):
+            for m in range(3):
+                A.check(A.lib().alto_mttkrp(T.h, m, RANK, uptr, outs[m].data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+        torch.cuda.synchronize()
+        return
 
     def run_als(prec, steps, warmup):
         sess = A.CpdSession(T, U0, precision=prec, stream=stream)
@@ -119,6 +143,9 @@ def main():
                               "fit": float(vfits[-1]), "fit_delta_vs_fp64": float(abs(vfits[-1] - fits[-1]))}
             log(f"{name} CP-ALS: {vdt / args.steps * 1e3:.3f} ms/iter, fit {vfits[-1]:.6f}")
 
+    apr = apr_leg() if args.apr else None
+    prec_study = precision_leg() if args.precision_study else None
+
     serving = None
     if args.serve_jobs > 0:
         serving = serve_leg(args.serve_jobs)
@@ -151,9 +178,112 @@ def main():
         "cpd_timers_s": tm,
         "precision_variants": variants,
         "serving": serving,
+        "cp_apr": apr,
+        "precision_study": prec_study,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def apr_leg():
+    """CP-APR on counts from a rank-10 Gamma(0.5) Kruskal model (about 120M
+    nonzeros), 128-bit ALTO keys, integer values, rank 10, 30 outer x 10 inner
+    iterations in bench mode (every inner iteration runs, as with -p)."""
+    import torch
+
+    import alto_amd as A
+    from alto_amd import gen
+
+    dims = [2 ** 20, 2 ** 18, 2 ** 16, 1024]
+    t0 = time.time()
+    coords, vals, _, _ = gen.poisson_kruskal_counts(dims, 10, 120_000_000, seed=3)
+    nnz = int(vals.numel())
+    T = A.AltoTensor.from_device(dims, [c.data_ptr() for c in coords], vals.data_ptr(), nnz, nprtn=16, lit_bits=128,
+                                 val_kind=A.VAL_U64)
+    del coords, vals
+    torch.cuda.empty_cache()
+    log(f"cp-apr: {nnz} nonzeros generated + ALTO in {time.time() - t0:.1f}s")
+    U, lam = A.kruskal_random_init(dims, 10, 3)
+    t = time.perf_counter()
+    r = T.cp_apr(U, lam, 30, 10, epsilon=0.0, eps_div_zero=0.0, dobench=True)
+    dt = time.perf_counter() - t
+    phi_bytes = nnz * (16 + 8) + sum(d * 10 * 8 for d in dims) + 0  # + I_n R 8 per mode (averaged below)
+    phi_calls = r["total_inner"]
+    per_phi = r["t_phi"] / max(phi_calls, 1)
+    avg_out = sum(d * 10 * 8 for d in dims) / len(dims)
+    phi_gbs = (nnz * (16 + 8) + sum(d * 10 * 8 for d in dims) + avg_out) / per_phi / 1e9
+    log(f"cp-apr: {dt:.2f}s total, {phi_calls} Phi calls, {per_phi * 1e3:.3f} ms/Phi ({phi_gbs:.0f} GB/s)")
+    del T
+    torch.cuda.empty_cache()
+    return {"dims": dims, "nnz": nnz, "rank": 10, "outer": r["outer_iters"], "inner_total": phi_calls,
+            "time_s": dt, "phi_time_s": r["t_phi"], "phi_ms_per_call": per_phi * 1e3, "phi_algorithmic_GBps": phi_gbs,
+            "loglik": r["loglik"], "ls_fit": r["ls_fit"], "kkt": r["kkt"]}
+
+
+def precision_leg():
+    """5-mode 50K x 50K x 10K x 2K x 512 tensor, 200M nonzeros, Zipf(1.2) per
+    mode, uniform (0,1) values, rank 64: per-mode MTTKRP time and relative error
+    of FP32 and BF16-storage (FP32 accumulation) against FP64, then 25 CP-ALS
+    iterations in each precision and the final-fit delta against FP64."""
+    import torch
+
+    import alto_amd as A
+    from alto_amd import gen
+
+    dims = [50000, 50000, 10000, 2000, 512]
+    R = 64
+    t0 = time.time()
+    coords, vals = gen.sparse_coo(dims, 200_000_000, 4, dist="zipf", s=1.2, values="uniform")
+    nnz = int(vals.numel())
+    T = A.AltoTensor.from_device(dims, [c.data_ptr() for c in coords], vals.data_ptr(), nnz, nprtn=16, lit_bits=128)
+    del coords, vals
+    torch.cuda.empty_cache()
+    log(f"precision: {nnz} nonzeros generated + ALTO in {time.time() - t0:.1f}s")
+    stream = torch.cuda.current_stream().cuda_stream
+    U0, _ = A.kruskal_random_init(dims, R, 4)
+    res = {"dims": dims, "nnz": nnz, "rank": R, "mttkrp": {}}
+    ref = {}
+    for name, prec in (("fp64", A.PREC_F64), ("fp32", A.PREC_F32), ("bf16", A.PREC_BF16)):
+        if prec == A.PREC_F64:
+            F = [torch.from_numpy(u).cuda() for u in U0]
+        elif prec == A.PREC_F32:
+            F = [torch.from_numpy(u).float().cuda() for u in U0]
+        else:
+            F = [torch.from_numpy(u).float().cuda().to(torch.bfloat16) for u in U0]
+        ptr = A._ptr_array([f.data_ptr() for f in F])
+        per = []
+        for m in range(len(dims)):
+            out = torch.zeros(dims[m], R, dtype=torch.float64 if prec == A.PREC_F64 else torch.float32, device="cuda")
+            A.check(A.lib().alto_mttkrp(T.h, m, R, ptr, out.data_ptr(), prec, 0, stream), "mttkrp")
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(3):
+                A.check(A.lib().alto_mttkrp(T.h, m, R, ptr, out.data_ptr(), prec, 0, stream), "mttkrp")
+            e1.record()
+            torch.cuda.synchronize()
+            o = out.double()
+            if prec == A.PREC_F64:
+                ref[m] = o
+                err = 0.0
+            else:
+                err = float(torch.linalg.norm(o - ref[m]) / torch.linalg.norm(ref[m]))
+            per.append({"ms": e0.elapsed_time(e1) / 3, "rel_err_vs_fp64": err})
+        res["mttkrp"][name] = per
+        del F
+    for name, prec in (("fp64", A.PREC_F64), ("fp32", A.PREC_F32), ("bf16", A.PREC_BF16)):
+        sess = A.CpdSession(T, U0, precision=prec, stream=stream)
+        t = time.perf_counter()
+        fits, _ = sess.iterate(25, epsilon=-1.0)
+        dt = time.perf_counter() - t
+        res[name + "_cpals"] = {"iters_per_s": 25 / dt, "final_fit": float(fits[-1])}
+        del sess
+    for name in ("fp32", "bf16"):
+        res[name + "_cpals"]["final_fit_delta_vs_fp64"] = abs(res[name + "_cpals"]["final_fit"] -
+                                                              res["fp64_cpals"]["final_fit"])
+    log(f"precision: {json.dumps(res)}")
+    del T
+    torch.cuda.empty_cache()
+    return res
 
 
 def serve_leg(n_jobs):

@@ -113,8 +113,11 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * (0 = default); streams are re-ordered on their next use.
  * ALTO_TUNE_NONTEMPORAL: 1 = stream the nonzeros with non-temporal loads.
  * ALTO_TUNE_MTTKRP_VARIANT: lane mapping of the MTTKRP kernel, 0..3 (2 strided
- * or 4 contiguous columns per lane; 4 or 8 nonzeros in flight per group). */
-enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4 };
+ * or 4 contiguous columns per lane; 4 or 8 nonzeros in flight per group).
+ * ALTO_TUNE_GRAPHS: 1 = CP-ALS sessions capture one iteration as a HIP graph
+ * and replay it (non-verbose iterate calls; phase timers are then not kept). */
+enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
+       ALTO_TUNE_GRAPHS = 5 };
 int alto_tune_set(int param, int value);
 
 /* ---- CP-ALS --------------------------------------------------------------- */

@@ -145,3 +145,30 @@ def test_session_split_iterations_equal_single_run():
     n, fit_all, U2, l2 = _als(dims, cidx, vals, 12, 7, seed=4)
     np.testing.assert_allclose(np.concatenate([f1, f2]), fit_all, rtol=1e-13)
     np.testing.assert_allclose(l1, l2, rtol=1e-12)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5)])
+def test_graph_replay_matches_eager(prec, tol):
+    """Iterations replayed from a captured HIP graph match eager launches; the
+    graph is replayed several times with eager work and syncs in between."""
+    A = _A()
+    dims = [70, 60, 50, 40]
+    cidx, vals = uniform_tensor(dims, 12000, 31)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    U, _ = A.kruskal_random_init(dims, 16, 5)
+    eager = A.CpdSession(g, U, precision=prec)
+    fe, _ = eager.iterate(7)
+    try:
+        A.check(A.lib().alto_tune_set(5, 1), "tune")
+        s = A.CpdSession(g, U, precision=prec)
+        f1, _ = s.iterate(2)
+        g.mttkrp(0, U)  # eager work between replays
+        f2, _ = s.iterate(3)
+        A.synchronize()
+        f3, _ = s.iterate(2)
+    finally:
+        A.lib().alto_tune_set(5, 0)
+    np.testing.assert_allclose(np.concatenate([f1, f2, f3]), fe, rtol=tol, atol=tol)
+    Ue, le, _, _ = eager.get()
+    Ug, lg, _, _ = s.get()
+    np.testing.assert_allclose(lg, le, rtol=1e3 * tol)
