@@ -76,6 +76,10 @@ struct ModeStream {
     DevBuf val64;   // nnz double (present when FP64 arithmetic was prepared)
     DevBuf val32;   // nnz float  (present when FP32/BF16 arithmetic was prepared)
     int chunk_log2 = -1;  // chunk size the stream was ordered with
+    // Secondary order inside a row: the nonzeros of one row are sorted by the
+    // row of mode `fib` (the "fiber" mode), so that consecutive nonzeros
+    // repeat that factor row; -1 = ALTO order inside the row, -2 = not built.
+    int fib = -2;
 };
 
 }  // namespace alto
@@ -118,5 +122,6 @@ int radix_sort_pairs(uint64_t* keys, int W, uint32_t* payload, uint64_t n, int b
 int delinearize_all(const alto_tensor* t, uint64_t* coords_dev, hipStream_t s);
 int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s);
 int tuned_chunk_log2();
+int fiber_order_enabled();
 
 }  // namespace alto

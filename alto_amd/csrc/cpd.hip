@@ -242,30 +242,35 @@ struct CpdSession : CpdSessionBase {
     // [1] later iterations (max-norm); the fit lands in pinned host memory
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     double* fit_host = nullptr;
+    // the graphs are recorded on a private stream (the caller's stream may
+    // be the null stream, which cannot be captured) and launched on `s`
+    hipStream_t cs = nullptr;
     ~CpdSession() override
     {
         for (auto& e : exec)
             if (e) (void)hipGraphExecDestroy(e);
         if (fit_host) (void)hipHostFree(fit_host);
+        if (cs) (void)hipStreamDestroy(cs);
     }
     int capture(int kind)
     {
         if (!fit_host) HIP_TRY(hipHostMalloc((void**)&fit_host, 32, hipHostMallocDefault));
+        if (!cs) HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         PhaseTimer off;
-        ALTO_TRY(off.init(false, s));
+        ALTO_TRY(off.init(false, cs));
         hipGraph_t g = nullptr;
-        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         int st = ALTO_OK;
-        for (int j = 0; j < c.N && st == ALTO_OK; ++j) st = cpd_mode_step(t, c, j, kind, off, s);
-        if (st == ALTO_OK) st = cpd_fit_step(t, c, normX, s);
+        for (int j = 0; j < c.N && st == ALTO_OK; ++j) st = cpd_mode_step(t, c, j, kind, off, cs);
+        if (st == ALTO_OK) st = cpd_fit_step(t, c, normX, cs);
         if (st == ALTO_OK) {
-            hipError_t e = hipMemcpyAsync(fit_host, c.fitbuf.p, 24, hipMemcpyDeviceToHost, s);
+            hipError_t e = hipMemcpyAsync(fit_host, c.fitbuf.p, 24, hipMemcpyDeviceToHost, cs);
             if (e != hipSuccess) {
                 set_error_hip(e, "hipMemcpyAsync (capture)");
                 st = ALTO_ERR_HIP;
             }
         }
-        hipError_t ec = hipStreamEndCapture(s, &g);
+        hipError_t ec = hipStreamEndCapture(cs, &g);
         if (st != ALTO_OK) {
             if (g) (void)hipGraphDestroy(g);
             return st;

@@ -684,7 +684,7 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ a, uint64_t n, unsig
 // per-mode streams: composite key (chunk << bits_mode) | row
 template <int W, int PW>
 __global__ void __launch_bounds__(256) k_mode_key(const uint64_t* __restrict__ keys, LutRef g_lut,
-                                                  PackLayout L, int mode, int chunk_log2, uint64_t n,
+                                                  PackLayout L, int mode, int fmode, int chunk_log2, uint64_t n,
                                                   uint64_t* __restrict__ ck, uint32_t* __restrict__ pay)
 {
     extern __shared__ uint64_t s_lut[];
@@ -695,7 +695,9 @@ __global__ void __launch_bounds__(256) k_mode_key(const uint64_t* __restrict__ k
         for (int w = 0; w < W; ++w) k[w] = keys[i * W + w];
         lut_apply<W, PW>(lut, g_lut.nb, k, pk);
         uint64_t r = field<PW>(pk, L.off[mode], L.bits[mode]);
-        ck[i] = ((i >> chunk_log2) << L.bits[mode]) | r;
+        uint64_t key = ((i >> chunk_log2) << L.bits[mode]) | r;
+        if (fmode >= 0) key = (key << L.bits[fmode]) | field<PW>(pk, L.off[fmode], L.bits[fmode]);
+        ck[i] = key;
         pay[i] = (uint32_t)i;
     }
 }
@@ -739,15 +741,31 @@ int delinearize_all(const alto_tensor* t, uint64_t* coords, hipStream_t s)
     return ALTO_OK;
 }
 
+// The fiber mode of target mode m: the other mode with the smallest
+// dimension (most repeated rows inside one output row), when the sort key
+// (chunk id, row, fiber row) fits 64 bits; -1 otherwise.
+static int fiber_mode_of(const alto_tensor* t, int m, int cbits)
+{
+    if (!fiber_order_enabled() || t->nmode < 2) return -1;
+    int f = -1;
+    for (int o = 0; o < t->nmode; ++o)
+        if (o != m && t->pack.bits[o] > 0 && (f < 0 || t->dims[o] < t->dims[f])) f = o;
+    if (f < 0 || cbits + t->pack.bits[m] + t->pack.bits[f] > 64) return -1;
+    return f;
+}
+
 int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
 {
     uint64_t n = t->nnz;
     const int want_chunk = tuned_chunk_log2();
     if (want_chunk > 0 && want_chunk != t->chunk_log2) t->chunk_log2 = want_chunk;
+    uint64_t nchunks = (n + (1ull << t->chunk_log2) - 1) >> t->chunk_log2;
+    int cbits = 0;
+    while ((1ull << cbits) < nchunks) ++cbits;
     bool any = false;
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
-        if (M.chunk_log2 != t->chunk_log2) {  // reorder for a new chunk size
+        if (M.chunk_log2 != t->chunk_log2 || M.fib != fiber_mode_of(t, m, cbits)) {  // reorder
             M.pk.release();
             M.val64.release();
             M.val32.release();
@@ -758,22 +776,24 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     DevBuf ck, pay;
     ALTO_TRY(ck.alloc(n * 8));
     ALTO_TRY(pay.alloc(n * 4));
-    uint64_t nchunks = (n + (1ull << t->chunk_log2) - 1) >> t->chunk_log2;
-    int cbits = 0;
-    while ((1ull << cbits) < nchunks) ++cbits;
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
+        if (M.pk.p && (!need64 || M.val64.p) && (!need32 || M.val32.p)) continue;
         M.chunk_log2 = t->chunk_log2;
+        M.fib = fiber_mode_of(t, m, cbits);
+        // an existing stream has this order already (the sort is deterministic):
+        // rewriting it leaves its other value array valid
         if (!M.pk.p) ALTO_TRY(M.pk.alloc(n * t->pack.pw * 8));
         if (need64 && !M.val64.p) ALTO_TRY(M.val64.alloc(n * 8));
         if (need32 && !M.val32.p) ALTO_TRY(M.val32.alloc(n * 4));
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s, t->key.as<uint64_t>(),
-                               lutref(t->delin), t->pack, m, t->chunk_log2, n, ck.as<uint64_t>(),
+                               lutref(t->delin), t->pack, m, M.fib, t->chunk_log2, n, ck.as<uint64_t>(),
                                pay.as<uint32_t>());
         });
         HIP_TRY(hipGetLastError());
-        ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, t->pack.bits[m] + cbits, s));
+        ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n,
+                                  t->pack.bits[m] + cbits + (M.fib >= 0 ? t->pack.bits[M.fib] : 0), s));
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_gather<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
                                t->key.as<uint64_t>(), t->vals.as<double>(), lutref(t->delin),
@@ -1027,7 +1047,7 @@ int alto_tensor_export_oidx(const alto_tensor_t* t, int mode, uint64_t* oidx)
     // rows with chunk_log2 = 63 -> chunk id 0 for every nonzero: a plain stable sort by row
     DISPATCH_WPW(t->W, t->pack.pw, {
         hipLaunchKernelGGL((k_mode_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), 0, t->key.as<uint64_t>(),
-                           lutref(t->delin), t->pack, mode, 63, n, ck.as<uint64_t>(), pay.as<uint32_t>());
+                           lutref(t->delin), t->pack, mode, -1, 63, n, ck.as<uint64_t>(), pay.as<uint32_t>());
     });
     HIP_TRY(hipGetLastError());
     ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, t->pack.bits[mode], 0));

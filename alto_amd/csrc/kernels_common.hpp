@@ -127,9 +127,13 @@ inline void fill_stream_args(const alto_tensor* t, int mode, int R, int seg, Str
 template <typename FT, typename VT, typename OT>
 inline void set_factors(const alto_tensor* t, int mode, const FT* const* U, StreamArgs<FT, VT, OT>& a)
 {
+    // the stream's fiber mode (its rows repeat between consecutive nonzeros)
+    // goes first, the other modes in natural order
+    const int f = t->ms[mode].fib;
     int o = 0;
-    for (int m = 0; m < t->nmode; ++m) {
-        if (m == mode) continue;
+    for (int k = -1; k < t->nmode; ++k) {
+        const int m = k < 0 ? f : k;
+        if (m < 0 || m == mode || (k >= 0 && m == f)) continue;
         a.Uo[o] = U[m];
         a.offo[o] = t->pack.off[m];
         a.bitso[o] = t->pack.bits[m];

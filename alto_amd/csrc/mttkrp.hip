@@ -30,9 +30,14 @@ int tuned_seg(int R)
 }
 static int g_chunk = 0;
 static int g_nt = 0;
-static int g_variant = 0;
+// -1: per element type, from measurements on MI355X at rank 32 (FP64: 2
+// strided columns / 4 in flight; FP32: 8 in flight; BF16: 4 contiguous
+// columns, 8-byte loads)
+static int g_variant = -1;
 static int g_graphs = 0;
+static int g_fiber = 1;
 int graphs_enabled() { return g_graphs; }
+int fiber_order_enabled() { return g_fiber; }
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
 {
@@ -134,7 +139,11 @@ __device__ inline T stream_ld(const T* p)
     else return *p;
 }
 
-template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J, bool NT, bool CONTIG>
+// FIB: the first other mode is the stream's fiber mode; a nonzero whose row
+// there equals the previous nonzero's reuses the row already in registers
+// instead of gathering it again.
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J, bool NT, bool CONTIG,
+          bool FIB>
 __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
 {
     const int lane = threadIdx.x & 63;
@@ -152,6 +161,12 @@ __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
 #pragma unroll
     for (int q = 0; q < E; ++q) acc[q] = 0;
     uint32_t cur = 0xffffffffu;
+    // last fiber-mode row offset and its values (row offsets are < 2^32 - R,
+    // so the initial value never matches)
+    uint32_t foff = 0xffffffffu;
+    AT fx[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) fx[q] = 0;
 
     for (uint64_t p = s; p < e; p += G) {
         // phase 0: lane gl decodes nonzero p+gl once (coalesced load)
@@ -186,8 +201,34 @@ __global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
 #pragma unroll
                 for (int q = 0; q < E; ++q) prod[jj][q] = v;
             }
+            if constexpr (FIB) {
+                // fiber mode: gather only rows that differ from the previous
+                // nonzero's (the predicate is uniform across the group)
+                const FT* __restrict__ U = a.Uo[0];
+                AT fxj[J][E];
+                uint32_t prev = foff;
 #pragma unroll
-            for (int o = 0; o < ALTO_MAXN - 1; ++o) {
+                for (int jj = 0; jj < J; ++jj) {
+                    const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+                    const uint32_t off = (uint32_t)__shfl((int)offs[0], j, G);
+                    if (off != prev) {
+                        load_cols<FT, AT, G, E, CONTIG>(U + off, gl, R, fxj[jj]);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < E; ++q) fxj[jj][q] = jj == 0 ? fx[q] : fxj[jj - 1][q];
+                    }
+                    prev = off;
+                }
+                foff = prev;
+#pragma unroll
+                for (int q = 0; q < E; ++q) fx[q] = fxj[J - 1][q];
+#pragma unroll
+                for (int jj = 0; jj < J; ++jj)
+#pragma unroll
+                    for (int q = 0; q < E; ++q) prod[jj][q] *= fxj[jj][q];
+            }
+#pragma unroll
+            for (int o = FIB ? 1 : 0; o < ALTO_MAXN - 1; ++o) {
                 if (o < nother) {
                     const FT* __restrict__ U = a.Uo[o];
 #pragma unroll
@@ -323,18 +364,21 @@ static int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const*
     // variants: 0 = 2 strided columns per lane, 4 nonzeros in flight per group;
     // 1 = same with 8 in flight; 2 = 4 contiguous columns per lane (16-byte
     // loads), 4 in flight; 3 = 4 contiguous columns, 8 in flight
-    const int v = g_variant;
+    const int v = g_variant >= 0 ? g_variant : std::is_same<FT, double>::value ? 0 : std::is_same<FT, float>::value ? 1 : 2;
     const int E = v >= 2 ? 4 : 2;
     int G = 4;
     while (G * E < R && G < 64) G <<= 1;
     dim3 grid = stream_grid(t->nnz, seg, G, a);
+    const bool fib = t->ms[mode].fib >= 0 && a.nother > 0;
 #define L_(GG, EE, JJ, CC)                                                                                      \
     do {                                                                                                         \
         if (t->pack.pw == 1) {                                                                                   \
-            if (g_nt) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, true, CC>), grid, dim3(256), 0, s, a); \
-            else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, false, CC>), grid, dim3(256), 0, s, a);     \
+            if (g_nt) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, true, CC, false>), grid, dim3(256), 0, s, a); \
+            else if (fib) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, false, CC, true>), grid, dim3(256), 0, s, a); \
+            else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, false, CC, false>), grid, dim3(256), 0, s, a); \
         } else {                                                                                                 \
-            hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 2, JJ, false, CC>), grid, dim3(256), 0, s, a);          \
+            if (fib) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 2, JJ, false, CC, true>), grid, dim3(256), 0, s, a); \
+            else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 2, JJ, false, CC, false>), grid, dim3(256), 0, s, a); \
         }                                                                                                        \
     } while (0)
 #define LG_(EE, JJ, CC)                                                                                          \
@@ -429,12 +473,16 @@ int alto_tune_set(int param, int value)
 {
     if (param == ALTO_TUNE_SEG) return set_seg(value);
     if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
+    if (param == ALTO_TUNE_FIBER_ORDER) {
+        g_fiber = value ? 1 : 0;
+        return ALTO_OK;
+    }
     if (param == ALTO_TUNE_GRAPHS) {
         g_graphs = value ? 1 : 0;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_MTTKRP_VARIANT) {
-        if (value < 0 || value > 3) return ALTO_ERR_ARG;
+        if (value < -1 || value > 3) return ALTO_ERR_ARG;
         g_variant = value;
         return ALTO_OK;
     }

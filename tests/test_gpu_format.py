@@ -138,3 +138,37 @@ def test_mttkrp_accumulate_and_zipf_heavy_rows():
         g.mttkrp_device(mode, dev, out, A.PREC_F64, accumulate=True)
         A.synchronize()
         np.testing.assert_allclose(out.numpy(), want + 1.0, rtol=1e-11)
+
+
+@pytest.fixture
+def tune_reset():
+    A = _alto()
+    yield A
+    for param, value in [(6, 1), (4, -1), (3, 0), (2, 0), (1, 0)]:
+        A.check(A.lib().alto_tune_set(param, value), "tune_set")
+
+
+@pytest.mark.parametrize("fiber", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("chunk_log2", [0, 10])
+def test_mttkrp_tuning_knobs_keep_results(tune_reset, fiber, variant, chunk_log2):
+    """Stream order inside a row (fiber order on/off), the lane mapping and
+    the chunk size change only the summation order: every combination matches
+    the oracle in FP64, FP32 and BF16, with the tolerances of the tests above."""
+    A = tune_reset
+    A.check(A.lib().alto_tune_set(6, fiber), "tune_set")
+    A.check(A.lib().alto_tune_set(4, variant), "tune_set")
+    A.check(A.lib().alto_tune_set(2, chunk_log2), "tune_set")
+    dims = [300, 40, 700, 25]
+    cidx, vals = zipf_tensor(dims, 30000, 17)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    o = oracle.OracleAlto(dims, cidx, vals, W=1, nprtn=8)
+    for R in (7, 32, 100):
+        rng = np.random.default_rng(R)
+        U = [rng.random((d, R)) for d in dims]
+        for mode in range(len(dims)):
+            want = o.mttkrp(mode, U)
+            np.testing.assert_allclose(g.mttkrp(mode, U), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+            assert _rel(g.mttkrp(mode, U, A.PREC_F32).astype(np.float64), want) < 1e-4
+            Ub = [A.from_bf16_bits(A.to_bf16_bits(u)).astype(np.float64) for u in U]
+            assert _rel(g.mttkrp(mode, U, A.PREC_BF16).astype(np.float64), mttkrp_numpy(cidx, vals, Ub, mode)) < 1e-4
