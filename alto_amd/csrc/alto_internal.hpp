@@ -76,10 +76,10 @@ struct ModeStream {
     DevBuf val64;   // nnz double (present when FP64 arithmetic was prepared)
     DevBuf val32;   // nnz float  (present when FP32/BF16 arithmetic was prepared)
     int chunk_log2 = -1;  // chunk size the stream was ordered with
-    // Secondary order inside a row: the nonzeros of one row are sorted by the
-    // row of mode `fib` (the "fiber" mode), so that consecutive nonzeros
-    // repeat that factor row; -1 = ALTO order inside the row, -2 = not built.
-    int fib = -2;
+    // 1: the fields of the other modes hold frequency ranks (rank 0 = the
+    // row with the most nonzeros) instead of rows; the kernels then read
+    // the other factors through row copies permuted into rank order
+    int ranked = 0;
 };
 
 }  // namespace alto
@@ -110,6 +110,15 @@ struct alto_tensor {
     int chunk_log2 = 20;      // log2 nonzeros per chunk of the per-mode streams
     alto::ModeStream ms[ALTO_MAXN];
     int device = 0;
+
+    // Frequency ranking of each mode's rows (descending nonzero count, ties
+    // by row): rank_row[m][k] = row of rank k, row_rank[m][r] = rank of row r.
+    // Present when the streams are ranked.
+    alto::DevBuf rank_row[ALTO_MAXN];
+    alto::DevBuf row_rank[ALTO_MAXN];
+    // factor rows permuted into rank order for the stream kernels, one buffer
+    // per mode (kept, grown on demand)
+    alto::DevBuf pcopy[ALTO_MAXN];
 };
 
 namespace alto {
@@ -122,6 +131,7 @@ int radix_sort_pairs(uint64_t* keys, int W, uint32_t* payload, uint64_t n, int b
 int delinearize_all(const alto_tensor* t, uint64_t* coords_dev, hipStream_t s);
 int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s);
 int tuned_chunk_log2();
-int fiber_order_enabled();
+int hot_rows_enabled();
+int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStream_t s);
 
 }  // namespace alto

@@ -223,7 +223,8 @@ static int loglik(alto_tensor* t, AprState& c, double eps, double* out, hipStrea
     fill_stream_args(t, 0, c.R, 1, a);
     const double* Uall[ALTO_MAXN];
     for (int m = 0; m < c.N; ++m) Uall[m] = c.U[m].as<double>();
-    set_factors(t, 0, Uall, a);
+    ALTO_TRY(reserve_rank_copies(t, (uint64_t)c.R, 8, s));
+    ALTO_TRY(set_factors(t, 0, Uall, a, s));
     a.val = t->ms[0].val64.as<double>();
     const int nb = 1024;
     RANK_DISPATCH(c.R, t->pack.pw, {

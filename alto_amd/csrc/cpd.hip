@@ -295,6 +295,8 @@ struct CpdSession : CpdSessionBase {
         s = st;
         c.prec = prec;
         ALTO_TRY(prepare_mode_streams(t, prec == ALTO_PREC_F64, prec != ALTO_PREC_F64, s));
+        // reserved here, outside any graph capture of the iterations
+        ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, 8, s));
         ALTO_TRY(cpd_alloc(t, R, c));
         ALTO_TRY(cpd_upload(t, c, Uh, s, dev_seed));
         ALTO_TRY(fill_value<double>(c.lambda.template as<double>(), R, 1.0, s));

@@ -684,7 +684,7 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ a, uint64_t n, unsig
 // per-mode streams: composite key (chunk << bits_mode) | row
 template <int W, int PW>
 __global__ void __launch_bounds__(256) k_mode_key(const uint64_t* __restrict__ keys, LutRef g_lut,
-                                                  PackLayout L, int mode, int fmode, int chunk_log2, uint64_t n,
+                                                  PackLayout L, int mode, int chunk_log2, uint64_t n,
                                                   uint64_t* __restrict__ ck, uint32_t* __restrict__ pay)
 {
     extern __shared__ uint64_t s_lut[];
@@ -695,9 +695,7 @@ __global__ void __launch_bounds__(256) k_mode_key(const uint64_t* __restrict__ k
         for (int w = 0; w < W; ++w) k[w] = keys[i * W + w];
         lut_apply<W, PW>(lut, g_lut.nb, k, pk);
         uint64_t r = field<PW>(pk, L.off[mode], L.bits[mode]);
-        uint64_t key = ((i >> chunk_log2) << L.bits[mode]) | r;
-        if (fmode >= 0) key = (key << L.bits[fmode]) | field<PW>(pk, L.off[fmode], L.bits[fmode]);
-        ck[i] = key;
+        ck[i] = ((i >> chunk_log2) << L.bits[mode]) | r;
         pay[i] = (uint32_t)i;
     }
 }
@@ -741,17 +739,146 @@ int delinearize_all(const alto_tensor* t, uint64_t* coords, hipStream_t s)
     return ALTO_OK;
 }
 
-// The fiber mode of target mode m: the other mode with the smallest
-// dimension (most repeated rows inside one output row), when the sort key
-// (chunk id, row, fiber row) fits 64 bits; -1 otherwise.
-static int fiber_mode_of(const alto_tensor* t, int m, int cbits)
+// ------------------------------------------------------- row frequency ranks
+// Nonzeros per row of mode m, from stream m (inside each chunk the rows are
+// contiguous): each run adds (end - start) through two wrapping additions.
+template <int PW>
+__global__ void __launch_bounds__(256) k_row_counts(const uint64_t* __restrict__ pk, uint64_t n, int off, int bits,
+                                                    int chunk_log2, uint32_t* __restrict__ cnt)
 {
-    if (!fiber_order_enabled() || t->nmode < 2) return -1;
-    int f = -1;
-    for (int o = 0; o < t->nmode; ++o)
-        if (o != m && t->pack.bits[o] > 0 && (f < 0 || t->dims[o] < t->dims[f])) f = o;
-    if (f < 0 || cbits + t->pack.bits[m] + t->pack.bits[f] > 64) return -1;
-    return f;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k[PW], kp[PW], kn[PW];
+#pragma unroll
+        for (int w = 0; w < PW; ++w) k[w] = pk[i * PW + w];
+        const uint64_t r = field<PW>(k, off, bits);
+        bool start = i == 0, end = i + 1 == n;
+        if (!start) {
+#pragma unroll
+            for (int w = 0; w < PW; ++w) kp[w] = pk[(i - 1) * PW + w];
+            start = field<PW>(kp, off, bits) != r || (i >> chunk_log2) != ((i - 1) >> chunk_log2);
+        }
+        if (!end) {
+#pragma unroll
+            for (int w = 0; w < PW; ++w) kn[w] = pk[(i + 1) * PW + w];
+            end = field<PW>(kn, off, bits) != r || (i >> chunk_log2) != ((i + 1) >> chunk_log2);
+        }
+        if (start) atomicAdd(&cnt[r], 0u - (uint32_t)i);
+        if (end) atomicAdd(&cnt[r], (uint32_t)(i + 1));
+    }
+}
+
+__global__ void k_rank_keys(const uint32_t* __restrict__ cnt, uint64_t d, int bits, uint64_t* __restrict__ keys,
+                            uint32_t* __restrict__ pay)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < d; r += (uint64_t)gridDim.x * blockDim.x) {
+        keys[r] = ((uint64_t)(0xffffffffu - cnt[r]) << bits) | r;
+        pay[r] = (uint32_t)r;
+    }
+}
+
+__global__ void k_rank_scatter(const uint32_t* __restrict__ pay, uint64_t d, uint32_t* __restrict__ rank_row,
+                               uint32_t* __restrict__ row_rank)
+{
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < d; k += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t r = pay[k];
+        rank_row[k] = r;
+        row_rank[r] = (uint32_t)k;
+    }
+}
+
+struct RankTabs {
+    const uint32_t* row_rank[ALTO_MAXN];
+};
+
+// rewrite the other modes' fields of one stream from rows to ranks
+template <int PW>
+__global__ void __launch_bounds__(256) k_stream_rank(uint64_t* __restrict__ pk, uint64_t n, PackLayout L, int mode,
+                                                     RankTabs T)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k[PW];
+#pragma unroll
+        for (int w = 0; w < PW; ++w) k[w] = pk[i * PW + w];
+        for (int o = 0; o < L.nmode; ++o) {
+            if (o == mode || L.bits[o] == 0) continue;
+            set_field<PW>(k, L.off[o], L.bits[o], T.row_rank[o][field<PW>(k, L.off[o], L.bits[o])]);
+        }
+#pragma unroll
+        for (int w = 0; w < PW; ++w) pk[i * PW + w] = k[w];
+    }
+}
+
+// rank tables from the (row-ordered) streams, then every unranked stream
+// rewritten; streams must all exist
+static int rank_streams(alto_tensor* t, hipStream_t s)
+{
+    const uint64_t n = t->nnz;
+    for (int m = 0; m < t->nmode; ++m) {
+        if (t->rank_row[m].p) continue;
+        const uint64_t d = t->dims[m];
+        DevBuf cnt, keys, pay;
+        ALTO_TRY(cnt.alloc(d * 4));
+        ALTO_TRY(keys.alloc(d * 8));
+        ALTO_TRY(pay.alloc(d * 4));
+        ALTO_TRY(t->rank_row[m].alloc(d * 4));
+        ALTO_TRY(t->row_rank[m].alloc(d * 4));
+        ALTO_TRY(fill_zero(cnt.as<uint32_t>(), d, s));
+        RANK_DISPATCH_PW(t->pack.pw, {
+            hipLaunchKernelGGL((k_row_counts<PW>), dim3(grid_for(n)), dim3(256), 0, s, t->ms[m].pk.as<uint64_t>(), n,
+                               t->pack.off[m], t->pack.bits[m], t->ms[m].chunk_log2, cnt.as<uint32_t>());
+        });
+        hipLaunchKernelGGL(k_rank_keys, dim3(grid_for(d)), dim3(256), 0, s, cnt.as<uint32_t>(), d, t->pack.bits[m],
+                           keys.as<uint64_t>(), pay.as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+        ALTO_TRY(radix_sort_pairs(keys.as<uint64_t>(), 1, pay.as<uint32_t>(), d, 32 + t->pack.bits[m], s));
+        hipLaunchKernelGGL(k_rank_scatter, dim3(grid_for(d)), dim3(256), 0, s, pay.as<uint32_t>(), d,
+                           t->rank_row[m].as<uint32_t>(), t->row_rank[m].as<uint32_t>());
+        HIP_TRY(hipGetLastError());
+        // the temporaries go back to the shared cache: drain first
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    RankTabs T;
+    for (int m = 0; m < ALTO_MAXN; ++m) T.row_rank[m] = m < t->nmode ? t->row_rank[m].as<uint32_t>() : nullptr;
+    for (int m = 0; m < t->nmode; ++m) {
+        ModeStream& M = t->ms[m];
+        if (M.ranked) continue;
+        RANK_DISPATCH_PW(t->pack.pw, {
+            hipLaunchKernelGGL((k_stream_rank<PW>), dim3(grid_for(n)), dim3(256), 0, s, M.pk.as<uint64_t>(), n,
+                               t->pack, m, T);
+        });
+        HIP_TRY(hipGetLastError());
+        M.ranked = 1;
+    }
+    return ALTO_OK;
+}
+
+int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStream_t s)
+{
+    bool ranked = false;
+    for (int m = 0; m < t->nmode; ++m) ranked = ranked || t->ms[m].ranked;
+    if (!ranked) return ALTO_OK;
+    bool drained = false;
+    for (int m = 0; m < t->nmode; ++m) {
+        const size_t need = t->dims[m] * rank * elem_bytes + 64;
+        if (t->pcopy[m].bytes >= need) continue;
+        if (!drained) {  // the old copy goes back to the shared cache
+            HIP_TRY(hipStreamSynchronize(s));
+            drained = true;
+        }
+        t->pcopy[m].release();
+        ALTO_TRY(t->pcopy[m].alloc(need));
+    }
+    return ALTO_OK;
+}
+
+// Ranked streams need every mode's rows below 2^32 (32-bit ranks) and a sort
+// key of 32 count bits + row bits.
+static bool rank_applicable(const alto_tensor* t)
+{
+    if (!hot_rows_enabled() || t->nmode < 2) return false;
+    for (int m = 0; m < t->nmode; ++m)
+        if (t->pack.bits[m] > 32) return false;
+    return true;
 }
 
 int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
@@ -762,17 +889,25 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     uint64_t nchunks = (n + (1ull << t->chunk_log2) - 1) >> t->chunk_log2;
     int cbits = 0;
     while ((1ull << cbits) < nchunks) ++cbits;
-    bool any = false;
+    const int ranked = (int)rank_applicable(t);
+    bool any = false, reorder = false;
+    for (int m = 0; m < t->nmode; ++m) {
+        const ModeStream& M = t->ms[m];
+        if (M.chunk_log2 != t->chunk_log2 || M.ranked != ranked) reorder = true;
+        if ((need64 && !M.val64.p) || (need32 && !M.val32.p) || !M.pk.p) any = true;
+    }
+    if (!any && !reorder) return ALTO_OK;
+    // buffers go back to the shared cache: drain work that may still read them
+    HIP_TRY(hipStreamSynchronize(s));
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
-        if (M.chunk_log2 != t->chunk_log2 || M.fib != fiber_mode_of(t, m, cbits)) {  // reorder
+        if (M.chunk_log2 != t->chunk_log2 || M.ranked != ranked) {
             M.pk.release();
             M.val64.release();
             M.val32.release();
+            M.ranked = 0;
         }
-        if ((need64 && !M.val64.p) || (need32 && !M.val32.p) || !M.pk.p) any = true;
     }
-    if (!any) return ALTO_OK;
     DevBuf ck, pay;
     ALTO_TRY(ck.alloc(n * 8));
     ALTO_TRY(pay.alloc(n * 4));
@@ -780,7 +915,7 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         ModeStream& M = t->ms[m];
         if (M.pk.p && (!need64 || M.val64.p) && (!need32 || M.val32.p)) continue;
         M.chunk_log2 = t->chunk_log2;
-        M.fib = fiber_mode_of(t, m, cbits);
+        M.ranked = 0;
         // an existing stream has this order already (the sort is deterministic):
         // rewriting it leaves its other value array valid
         if (!M.pk.p) ALTO_TRY(M.pk.alloc(n * t->pack.pw * 8));
@@ -788,12 +923,11 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         if (need32 && !M.val32.p) ALTO_TRY(M.val32.alloc(n * 4));
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s, t->key.as<uint64_t>(),
-                               lutref(t->delin), t->pack, m, M.fib, t->chunk_log2, n, ck.as<uint64_t>(),
+                               lutref(t->delin), t->pack, m, t->chunk_log2, n, ck.as<uint64_t>(),
                                pay.as<uint32_t>());
         });
         HIP_TRY(hipGetLastError());
-        ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n,
-                                  t->pack.bits[m] + cbits + (M.fib >= 0 ? t->pack.bits[M.fib] : 0), s));
+        ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, t->pack.bits[m] + cbits, s));
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_gather<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
                                t->key.as<uint64_t>(), t->vals.as<double>(), lutref(t->delin),
@@ -802,6 +936,7 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         });
         HIP_TRY(hipGetLastError());
     }
+    if (rank_applicable(t)) ALTO_TRY(rank_streams(t, s));
     HIP_TRY(hipStreamSynchronize(s));
     return ALTO_OK;
 }
@@ -1047,7 +1182,7 @@ int alto_tensor_export_oidx(const alto_tensor_t* t, int mode, uint64_t* oidx)
     // rows with chunk_log2 = 63 -> chunk id 0 for every nonzero: a plain stable sort by row
     DISPATCH_WPW(t->W, t->pack.pw, {
         hipLaunchKernelGGL((k_mode_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), 0, t->key.as<uint64_t>(),
-                           lutref(t->delin), t->pack, mode, -1, 63, n, ck.as<uint64_t>(), pay.as<uint32_t>());
+                           lutref(t->delin), t->pack, mode, 63, n, ck.as<uint64_t>(), pay.as<uint32_t>());
     });
     HIP_TRY(hipGetLastError());
     ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, t->pack.bits[mode], 0));

@@ -46,6 +46,27 @@ __device__ inline uint64_t field(const uint64_t (&pk)[PW], int off, int bits)
     return v & msk;
 }
 
+template <int PW>
+__device__ inline void set_field(uint64_t (&pk)[PW], int off, int bits, uint64_t v)
+{
+    if (bits == 0) return;
+    const uint64_t msk = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+    v &= msk;
+    if (PW == 1) {
+        pk[0] = (pk[0] & ~(msk << off)) | (v << off);
+        return;
+    }
+    if (off >= 64) {
+        pk[PW - 1] = (pk[PW - 1] & ~(msk << (off - 64))) | (v << (off - 64));
+        return;
+    }
+    pk[0] = (pk[0] & ~(msk << off)) | (v << off);
+    if (off + bits > 64) {  // the high part continues in word 1 (off > 0 here)
+        const uint64_t hm = (1ull << (off + bits - 64)) - 1;
+        pk[PW - 1] = (pk[PW - 1] & ~hm) | (v >> (64 - off));
+    }
+}
+
 __device__ inline uint64_t shfl_u64(uint64_t x, int src, int width)
 {
     uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
@@ -102,6 +123,10 @@ struct StreamArgs {
     uint64_t seg;
     uint64_t nblocks_real;
     int nmode, mode, rank;
+    // LDS staging of hot rows (ranked streams): elements of Uo[o] staged and
+    // their byte offset in the workgroup's LDS
+    uint32_t hot_n[ALTO_MAXN - 1];
+    uint32_t hot_base[ALTO_MAXN - 1];
 };
 
 template <typename FT, typename VT, typename OT>
@@ -113,6 +138,7 @@ inline void fill_stream_args(const alto_tensor* t, int mode, int R, int seg, Str
     for (int o = 0; o < ALTO_MAXN - 1; ++o) {
         a.Uo[o] = nullptr;
         a.offo[o] = a.bitso[o] = 0;
+        a.hot_n[o] = a.hot_base[o] = 0;
     }
     a.nother = 0;
     a.offn = t->pack.off[mode];
@@ -124,22 +150,70 @@ inline void fill_stream_args(const alto_tensor* t, int mode, int R, int seg, Str
     a.rank = R;
 }
 
-template <typename FT, typename VT, typename OT>
-inline void set_factors(const alto_tensor* t, int mode, const FT* const* U, StreamArgs<FT, VT, OT>& a)
+// Factor rows in rank order for a ranked stream: dst row k = src row rank_row[k].
+template <typename FT>
+__global__ void k_permute_rows(const FT* __restrict__ src, const uint32_t* __restrict__ rank_row, uint64_t d, int R,
+                               FT* __restrict__ dst)
 {
-    // the stream's fiber mode (its rows repeat between consecutive nonzeros)
-    // goes first, the other modes in natural order
-    const int f = t->ms[mode].fib;
+    const uint64_t n = d * (uint64_t)R;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = i / (uint64_t)R;
+        dst[i] = src[(uint64_t)rank_row[k] * R + (i - k * R)];
+    }
+}
+
+// Points the stream arguments at the other modes' factors (natural mode
+// order).  For a ranked stream the kernels index the factors by rank: the
+// rows are first copied into rank order (into buffers reserved by
+// reserve_rank_copies on this stream's tensor).
+template <typename FT, typename VT, typename OT>
+inline int set_factors(const alto_tensor* t, int mode, const FT* const* U, StreamArgs<FT, VT, OT>& a, hipStream_t s)
+{
+    const bool ranked = t->ms[mode].ranked != 0;
     int o = 0;
-    for (int k = -1; k < t->nmode; ++k) {
-        const int m = k < 0 ? f : k;
-        if (m < 0 || m == mode || (k >= 0 && m == f)) continue;
+    for (int m = 0; m < t->nmode; ++m) {
+        if (m == mode) continue;
         a.Uo[o] = U[m];
+        if (ranked) {
+            const uint64_t n = t->dims[m] * (uint64_t)a.rank;
+            if (t->pcopy[m].bytes < n * sizeof(FT) + 64) {
+                set_error("rank-ordered factor copies not reserved (mode %d)", m);
+                return ALTO_ERR_UNSUPPORTED;
+            }
+            uint64_t b = (n + 255) / 256;
+            if (b > 8192) b = 8192;
+            hipLaunchKernelGGL(k_permute_rows<FT>, dim3((unsigned)b), dim3(256), 0, s, U[m],
+                               t->rank_row[m].as<uint32_t>(), t->dims[m], a.rank, t->pcopy[m].as<FT>());
+            HIP_TRY(hipGetLastError());
+            a.Uo[o] = t->pcopy[m].as<FT>();
+        }
         a.offo[o] = t->pack.off[m];
         a.bitso[o] = t->pack.bits[m];
         ++o;
     }
     a.nother = o;
+    return ALTO_OK;
+}
+
+// LDS bytes of hot rows per workgroup of the ranked-stream kernels, split
+// evenly over the other modes; returns the bytes used.
+template <typename FT, typename VT, typename OT>
+inline uint32_t plan_hot_rows(const alto_tensor* t, int mode, StreamArgs<FT, VT, OT>& a, uint32_t lds_bytes)
+{
+    if (!t->ms[mode].ranked || a.nother == 0) return 0;
+    const uint64_t per = (lds_bytes / a.nother) & ~15ull;  // 16-byte aligned regions
+    uint32_t base = 0;
+    int o = 0;
+    for (int m = 0; m < t->nmode; ++m) {
+        if (m == mode) continue;
+        uint64_t rows = per / ((uint64_t)a.rank * sizeof(FT));
+        if (rows > t->dims[m]) rows = t->dims[m];
+        a.hot_n[o] = (uint32_t)(rows * a.rank);
+        a.hot_base[o] = base;
+        base += (uint32_t)((rows * a.rank * sizeof(FT) + 15) / 16 * 16);
+        ++o;
+    }
+    return base;
 }
 
 // lanes per nonzero: 2 strided columns per lane, 4..64 lanes
@@ -148,11 +222,11 @@ int tuned_seg(int R);
 inline int default_seg(int R) { return tuned_seg(R); }
 
 template <typename A>
-inline dim3 stream_grid(uint64_t nnz, int seg, int G, A& a)
+inline dim3 stream_grid(uint64_t nnz, int seg, int G, A& a, int block_threads = 256)
 {
     uint64_t groups = (nnz + seg - 1) / seg;
     uint64_t threads = groups * G;
-    uint64_t blocks = (threads + 255) / 256;
+    uint64_t blocks = (threads + block_threads - 1) / block_threads;
     if (blocks < 1) blocks = 1;
     a.nblocks_real = blocks;
     uint64_t nb8 = (blocks + 7) / 8;

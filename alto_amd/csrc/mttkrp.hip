@@ -1,4 +1,5 @@
-// MTTKRP and CP-APR Phi over the per-mode chunk-ordered nonzero streams.
+// MTTKRP (host side; kernels in mttkrp_kernels.hpp) and CP-APR Phi over the
+// per-mode chunk-ordered nonzero streams.
 //
 // Layout: the nonzeros of each ALTO chunk (2^chunk_log2 consecutive nonzeros
 // in ALTO order) are stored, per target mode, sorted by that mode's row
@@ -15,8 +16,7 @@
 // Reference semantics: mttkrp_alto_par / mttkrp_alto_da_mem_pull
 // (include/alto.hpp:617-623, :1227-1318) and CalculatePhi_alto_par
 // (include/apr.hpp:295-303, :383-501).
-#include "alto_internal.hpp"
-#include "kernels_common.hpp"
+#include "mttkrp_kernels.hpp"
 
 namespace alto {
 
@@ -30,14 +30,14 @@ int tuned_seg(int R)
 }
 static int g_chunk = 0;
 static int g_nt = 0;
-// -1: per element type, from measurements on MI355X at rank 32 (FP64: 2
-// strided columns / 4 in flight; FP32: 8 in flight; BF16: 4 contiguous
-// columns, 8-byte loads)
+// lane mapping of the MTTKRP kernel; -1 = per element type
 static int g_variant = -1;
 static int g_graphs = 0;
-static int g_fiber = 1;
+static int g_hot = 0;
+static int g_hot_block = 512;
+static int g_hot_lds = 49152;
 int graphs_enabled() { return g_graphs; }
-int fiber_order_enabled() { return g_fiber; }
+int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
 {
@@ -50,215 +50,6 @@ int set_seg(int v)
     if (v < 0 || v > (1 << 20)) return ALTO_ERR_ARG;
     g_seg = v;
     return ALTO_OK;
-}
-
-// Column mapping of a lane group: strided (lane gl holds c = gl + q*G, so
-// each gather / atomic instruction touches G contiguous elements) or
-// contiguous (lane gl holds c = gl*E + q, loaded with one 16-byte load where
-// the row allows it).
-template <bool CONTIG, int G, int E>
-__device__ inline int col_of(int gl, int q) { return CONTIG ? gl * E + q : gl + q * G; }
-
-template <typename FT, int E> struct VecLoad {
-    static constexpr bool ok = false;
-};
-template <> struct VecLoad<double, 2> {
-    static constexpr bool ok = true;
-    template <typename AT> __device__ static void load(const double* p, AT (&x)[2])
-    {
-        const double2 a = *reinterpret_cast<const double2*>(p);
-        x[0] = (AT)a.x;
-        x[1] = (AT)a.y;
-    }
-};
-template <> struct VecLoad<double, 4> {
-    static constexpr bool ok = true;
-    template <typename AT> __device__ static void load(const double* p, AT (&x)[4])
-    {
-        const double2 a = *reinterpret_cast<const double2*>(p);
-        const double2 b = *reinterpret_cast<const double2*>(p + 2);
-        x[0] = (AT)a.x; x[1] = (AT)a.y; x[2] = (AT)b.x; x[3] = (AT)b.y;
-    }
-};
-template <> struct VecLoad<float, 4> {
-    static constexpr bool ok = true;
-    template <typename AT> __device__ static void load(const float* p, AT (&x)[4])
-    {
-        const float4 a = *reinterpret_cast<const float4*>(p);
-        x[0] = (AT)a.x; x[1] = (AT)a.y; x[2] = (AT)a.z; x[3] = (AT)a.w;
-    }
-};
-template <> struct VecLoad<float, 2> {
-    static constexpr bool ok = true;
-    template <typename AT> __device__ static void load(const float* p, AT (&x)[2])
-    {
-        const float2 a = *reinterpret_cast<const float2*>(p);
-        x[0] = (AT)a.x; x[1] = (AT)a.y;
-    }
-};
-template <> struct VecLoad<bf16_t, 4> {
-    static constexpr bool ok = true;
-    template <typename AT> __device__ static void load(const bf16_t* p, AT (&x)[4])
-    {
-        const uint2 a = *reinterpret_cast<const uint2*>(p);
-        x[0] = (AT)__uint_as_float(a.x << 16); x[1] = (AT)__uint_as_float(a.x & 0xffff0000u);
-        x[2] = (AT)__uint_as_float(a.y << 16); x[3] = (AT)__uint_as_float(a.y & 0xffff0000u);
-    }
-};
-template <> struct VecLoad<bf16_t, 2> {
-    static constexpr bool ok = true;
-    template <typename AT> __device__ static void load(const bf16_t* p, AT (&x)[2])
-    {
-        const uint32_t a = *reinterpret_cast<const uint32_t*>(p);
-        x[0] = (AT)__uint_as_float(a << 16); x[1] = (AT)__uint_as_float(a & 0xffff0000u);
-    }
-};
-
-template <typename FT, typename AT, int G, int E, bool CONTIG = false>
-__device__ inline void load_cols(const FT* __restrict__ row, int gl, int R, AT (&x)[E])
-{
-    if constexpr (CONTIG && VecLoad<FT, E>::ok) {
-        const int c0 = gl * E;
-        // bf16 rows of odd rank are only 2-byte aligned: element loads there
-        if (c0 + E <= R && (sizeof(FT) >= 4 || (R & 1) == 0)) {
-            VecLoad<FT, E>::load(row + c0, x);
-            return;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-        const int c = col_of<CONTIG, G, E>(gl, q);
-        x[q] = c < R ? to_acc<AT>(row[c]) : (AT)0;
-    }
-}
-
-template <bool NT, typename T>
-__device__ inline T stream_ld(const T* p)
-{
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-
-// FIB: the first other mode is the stream's fiber mode; a nonzero whose row
-// there equals the previous nonzero's reuses the row already in registers
-// instead of gathering it again.
-template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int J, bool NT, bool CONTIG,
-          bool FIB>
-__global__ void __launch_bounds__(256) k_mttkrp(StreamArgs<FT, VT, OT> a)
-{
-    const int lane = threadIdx.x & 63;
-    const int gl = lane & (G - 1);
-    const uint64_t block = xcd_block(a.nblocks_real);
-    if (block == ~0ull) return;
-    const uint64_t group = (block * 256 + threadIdx.x) / G;
-    const uint64_t s = group * a.seg;
-    if (s >= a.nnz) return;
-    const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
-    const int R = a.rank;
-    const int nother = a.nother;
-
-    AT acc[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q) acc[q] = 0;
-    uint32_t cur = 0xffffffffu;
-    // last fiber-mode row offset and its values (row offsets are < 2^32 - R,
-    // so the initial value never matches)
-    uint32_t foff = 0xffffffffu;
-    AT fx[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q) fx[q] = 0;
-
-    for (uint64_t p = s; p < e; p += G) {
-        // phase 0: lane gl decodes nonzero p+gl once (coalesced load)
-        uint32_t offs[ALTO_MAXN - 1];
-        uint32_t rown = 0;
-        VT myv = 0;
-        {
-            uint64_t k[PW];
-            if (p + gl < e) {
-#pragma unroll
-                for (int w = 0; w < PW; ++w) k[w] = stream_ld<NT>(&a.pk[(p + gl) * PW + w]);
-                myv = stream_ld<NT>(&a.val[p + gl]);
-            } else {
-#pragma unroll
-                for (int w = 0; w < PW; ++w) k[w] = 0;
-            }
-#pragma unroll
-            for (int o = 0; o < ALTO_MAXN - 1; ++o)
-                offs[o] = o < nother ? (uint32_t)(field<PW>(k, a.offo[o], a.bitso[o]) * (uint64_t)R) : 0u;
-            rown = (uint32_t)field<PW>(k, a.offn, a.bitsn);
-        }
-        const int cnt = (int)(e - p < (uint64_t)G ? e - p : (uint64_t)G);
-        for (int j0 = 0; j0 < cnt; j0 += J) {
-            // phase 1: broadcast J nonzeros, issue all their row gathers
-            AT prod[J][E];
-            uint32_t rj[J];
-#pragma unroll
-            for (int jj = 0; jj < J; ++jj) {
-                const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
-                rj[jj] = (uint32_t)__shfl((int)rown, j, G);
-                const AT v = (AT)shfl_val(myv, j, G);
-#pragma unroll
-                for (int q = 0; q < E; ++q) prod[jj][q] = v;
-            }
-            if constexpr (FIB) {
-                // fiber mode: gather only rows that differ from the previous
-                // nonzero's (the predicate is uniform across the group)
-                const FT* __restrict__ U = a.Uo[0];
-                AT fxj[J][E];
-                uint32_t prev = foff;
-#pragma unroll
-                for (int jj = 0; jj < J; ++jj) {
-                    const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
-                    const uint32_t off = (uint32_t)__shfl((int)offs[0], j, G);
-                    if (off != prev) {
-                        load_cols<FT, AT, G, E, CONTIG>(U + off, gl, R, fxj[jj]);
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < E; ++q) fxj[jj][q] = jj == 0 ? fx[q] : fxj[jj - 1][q];
-                    }
-                    prev = off;
-                }
-                foff = prev;
-#pragma unroll
-                for (int q = 0; q < E; ++q) fx[q] = fxj[J - 1][q];
-#pragma unroll
-                for (int jj = 0; jj < J; ++jj)
-#pragma unroll
-                    for (int q = 0; q < E; ++q) prod[jj][q] *= fxj[jj][q];
-            }
-#pragma unroll
-            for (int o = FIB ? 1 : 0; o < ALTO_MAXN - 1; ++o) {
-                if (o < nother) {
-                    const FT* __restrict__ U = a.Uo[o];
-#pragma unroll
-                    for (int jj = 0; jj < J; ++jj) {
-                        const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
-                        const uint32_t off = (uint32_t)__shfl((int)offs[o], j, G);
-                        AT x[E];
-                        load_cols<FT, AT, G, E, CONTIG>(U + off, gl, R, x);
-#pragma unroll
-                        for (int q = 0; q < E; ++q) prod[jj][q] *= x[q];
-                    }
-                }
-            }
-            // phase 2: register reduction while the output row repeats
-#pragma unroll
-            for (int jj = 0; jj < J; ++jj) {
-                if (j0 + jj < cnt) {
-                    if (rj[jj] != cur) {
-                        if (cur != 0xffffffffu) flush_row<AT, OT, G, E, CONTIG>(a.out, cur, R, gl, acc);
-                        cur = rj[jj];
-#pragma unroll
-                        for (int q = 0; q < E; ++q) acc[q] = 0;
-                    }
-#pragma unroll
-                    for (int q = 0; q < E; ++q) acc[q] += prod[jj][q];
-                }
-            }
-        }
-    }
-    if (cur != 0xffffffffu) flush_row<AT, OT, G, E, CONTIG>(a.out, cur, R, gl, acc);
 }
 
 // Phi: per nonzero pi = prod_{m != n} U_m[row_m,:], d = <B[row,:], pi>,
@@ -352,57 +143,6 @@ __global__ void __launch_bounds__(256) k_phi(StreamArgs<double, double, double> 
 }
 
 // ------------------------------------------------------------------ host side
-template <typename FT, typename VT, typename AT, typename OT>
-static int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, const VT* val, OT* out, int seg,
-                         hipStream_t s)
-{
-    StreamArgs<FT, VT, OT> a;
-    fill_stream_args(t, mode, R, seg, a);
-    set_factors(t, mode, U, a);
-    a.val = val;
-    a.out = out;
-    // variants: 0 = 2 strided columns per lane, 4 nonzeros in flight per group;
-    // 1 = same with 8 in flight; 2 = 4 contiguous columns per lane (16-byte
-    // loads), 4 in flight; 3 = 4 contiguous columns, 8 in flight
-    const int v = g_variant >= 0 ? g_variant : std::is_same<FT, double>::value ? 0 : std::is_same<FT, float>::value ? 1 : 2;
-    const int E = v >= 2 ? 4 : 2;
-    int G = 4;
-    while (G * E < R && G < 64) G <<= 1;
-    dim3 grid = stream_grid(t->nnz, seg, G, a);
-    const bool fib = t->ms[mode].fib >= 0 && a.nother > 0;
-#define L_(GG, EE, JJ, CC)                                                                                      \
-    do {                                                                                                         \
-        if (t->pack.pw == 1) {                                                                                   \
-            if (g_nt) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, true, CC, false>), grid, dim3(256), 0, s, a); \
-            else if (fib) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, false, CC, true>), grid, dim3(256), 0, s, a); \
-            else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 1, JJ, false, CC, false>), grid, dim3(256), 0, s, a); \
-        } else {                                                                                                 \
-            if (fib) hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 2, JJ, false, CC, true>), grid, dim3(256), 0, s, a); \
-            else hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, 2, JJ, false, CC, false>), grid, dim3(256), 0, s, a); \
-        }                                                                                                        \
-    } while (0)
-#define LG_(EE, JJ, CC)                                                                                          \
-    do {                                                                                                         \
-        switch (G) {                                                                                             \
-        case 4: L_(4, EE, JJ, CC); break;                                                                        \
-        case 8: L_(8, EE, JJ, CC); break;                                                                        \
-        case 16: L_(16, EE, JJ, CC); break;                                                                      \
-        case 32: L_(32, EE, JJ, CC); break;                                                                      \
-        default: L_(64, EE, JJ, CC); break;                                                                      \
-        }                                                                                                        \
-    } while (0)
-    switch (v) {
-    case 1: LG_(2, 8, false); break;
-    case 2: LG_(4, 4, true); break;
-    case 3: LG_(4, 8, true); break;
-    default: LG_(2, 4, false); break;
-    }
-#undef LG_
-#undef L_
-    HIP_TRY(hipGetLastError());
-    return ALTO_OK;
-}
-
 // factor row offsets (row * rank) are carried as 32-bit values
 static int check_offsets(const alto_tensor* t, int R)
 {
@@ -426,18 +166,24 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
     ALTO_TRY(prepare_mode_streams(t, precision == ALTO_PREC_F64, precision != ALTO_PREC_F64, s));
     uint64_t nout = t->dims[mode] * (uint64_t)R;
     if (seg <= 0) seg = default_seg(R);
+    ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, precision == ALTO_PREC_F64 ? 8 : precision == ALTO_PREC_F32 ? 4 : 2, s));
+    MttkrpTune k;
+    k.variant = g_variant;
+    k.nt = g_nt;
+    k.hot_block = g_hot_block;
+    k.hot_lds = g_hot_lds;
     if (precision == ALTO_PREC_F64) {
         if (!accumulate) ALTO_TRY(fill_zero((double*)out, nout, s));
-        return launch_mttkrp<double, double, double, double>(t, mode, R, (const double* const*)U,
-                                                             t->ms[mode].val64.as<double>(), (double*)out, seg, s);
+        return launch_mttkrp_f64(t, mode, R, (const double* const*)U, t->ms[mode].val64.as<double>(), (double*)out, seg,
+                                 k, s);
     }
     if (!accumulate) ALTO_TRY(fill_zero((float*)out, nout, s));
     if (precision == ALTO_PREC_F32)
-        return launch_mttkrp<float, float, float, float>(t, mode, R, (const float* const*)U, t->ms[mode].val32.as<float>(),
-                                                         (float*)out, seg, s);
+        return launch_mttkrp_f32(t, mode, R, (const float* const*)U, t->ms[mode].val32.as<float>(), (float*)out, seg, k,
+                                 s);
     if (precision == ALTO_PREC_BF16)
-        return launch_mttkrp<bf16_t, float, float, float>(t, mode, R, (const bf16_t* const*)U,
-                                                          t->ms[mode].val32.as<float>(), (float*)out, seg, s);
+        return launch_mttkrp_bf16(t, mode, R, (const bf16_t* const*)U, t->ms[mode].val32.as<float>(), (float*)out, seg,
+                                  k, s);
     set_error("mttkrp: unknown precision %d", precision);
     return ALTO_ERR_ARG;
 }
@@ -452,9 +198,10 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
     ALTO_TRY(prepare_mode_streams(t, 1, 0, s));
     if (seg <= 0) seg = default_seg(R);
     ALTO_TRY(fill_zero(phi, t->dims[mode] * (uint64_t)R, s));
+    ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, 8, s));
     StreamArgs<double, double, double> a;
     fill_stream_args(t, mode, R, seg, a);
-    set_factors(t, mode, U, a);
+    ALTO_TRY(set_factors(t, mode, U, a, s));
     a.val = t->ms[mode].val64.as<double>();
     a.out = phi;
     dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
@@ -473,8 +220,18 @@ int alto_tune_set(int param, int value)
 {
     if (param == ALTO_TUNE_SEG) return set_seg(value);
     if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
-    if (param == ALTO_TUNE_FIBER_ORDER) {
-        g_fiber = value ? 1 : 0;
+    if (param == ALTO_TUNE_HOT_ROWS) {
+        g_hot = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_HOT_BLOCK) {
+        if (value != 256 && value != 512 && value != 1024) return ALTO_ERR_ARG;
+        g_hot_block = value;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_HOT_LDS) {
+        if (value < 0 || value > 160 * 1024) return ALTO_ERR_ARG;
+        g_hot_lds = value;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_GRAPHS) {
@@ -482,7 +239,7 @@ int alto_tune_set(int param, int value)
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_MTTKRP_VARIANT) {
-        if (value < -1 || value > 3) return ALTO_ERR_ARG;
+        if (value < -1 || value > 2) return ALTO_ERR_ARG;
         g_variant = value;
         return ALTO_OK;
     }

@@ -1,0 +1,528 @@
+// MTTKRP stream kernel and its launcher, instantiated per factor element
+// type in mttkrp_f64.hip / mttkrp_f32.hip / mttkrp_bf16.hip (one translation
+// unit each, so that they build in parallel).
+//
+// Reference semantics: mttkrp_alto_par / mttkrp_alto_da_mem_pull
+// (include/alto.hpp:617-623, :1227-1318).
+#pragma once
+
+#include <type_traits>
+
+#include "alto_internal.hpp"
+#include "kernels_common.hpp"
+
+namespace alto {
+
+// launch configuration chosen by the host side (tunables)
+struct MttkrpTune {
+    int variant = -1;       // lane mapping 0..3, -1 = per element type
+    int nt = 0;             // non-temporal stream loads
+    int hot_block = 512;    // threads per workgroup of the ranked-stream kernel
+    int hot_lds = 49152;    // LDS bytes per workgroup for hot rows
+};
+
+// Column mapping of a lane group: strided (lane gl holds c = gl + q*G, so
+// each gather / atomic instruction touches G contiguous elements) or
+// contiguous (lane gl holds c = gl*E + q, loaded with one 16-byte load where
+// the row allows it).
+template <bool CONTIG, int G, int E>
+__device__ inline int col_of(int gl, int q) { return CONTIG ? gl * E + q : gl + q * G; }
+
+// Row loads from global memory (plain pointers) or from LDS (address space
+// 3 pointers, so that the compiler emits ds_read rather than flat loads).
+template <typename T> using lds_ptr = const __attribute__((address_space(3))) T*;
+typedef double v2f64 __attribute__((ext_vector_type(2)));
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+typedef float v2f32 __attribute__((ext_vector_type(2)));
+typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
+
+template <typename V, typename T> __device__ inline const V* vcast(const T* p) { return reinterpret_cast<const V*>(p); }
+template <typename V, typename T> __device__ inline lds_ptr<V> vcast(lds_ptr<T> p)
+{
+    return reinterpret_cast<lds_ptr<V>>(p);
+}
+
+template <typename AT, typename P> __device__ inline AT elem_acc(P p, int c, double*) { return (AT)p[c]; }
+template <typename AT, typename P> __device__ inline AT elem_acc(P p, int c, float*) { return (AT)p[c]; }
+template <typename AT, typename P> __device__ inline AT elem_acc(P p, int c, bf16_t*)
+{
+    return (AT)__uint_as_float((uint32_t)p[c].x << 16);
+}
+
+template <typename FT, int E> struct VecLoad {
+    static constexpr bool ok = false;
+};
+template <> struct VecLoad<double, 2> {
+    static constexpr bool ok = true;
+    template <typename AT, typename P> __device__ static void load(P p, AT (&x)[2])
+    {
+        const v2f64 a = *vcast<v2f64>(p);
+        x[0] = (AT)a.x;
+        x[1] = (AT)a.y;
+    }
+};
+template <> struct VecLoad<double, 4> {
+    static constexpr bool ok = true;
+    template <typename AT, typename P> __device__ static void load(P p, AT (&x)[4])
+    {
+        const v2f64 a = *vcast<v2f64>(p);
+        const v2f64 b = *vcast<v2f64>(p + 2);
+        x[0] = (AT)a.x; x[1] = (AT)a.y; x[2] = (AT)b.x; x[3] = (AT)b.y;
+    }
+};
+template <> struct VecLoad<float, 4> {
+    static constexpr bool ok = true;
+    template <typename AT, typename P> __device__ static void load(P p, AT (&x)[4])
+    {
+        const v4f32 a = *vcast<v4f32>(p);
+        x[0] = (AT)a.x; x[1] = (AT)a.y; x[2] = (AT)a.z; x[3] = (AT)a.w;
+    }
+};
+template <> struct VecLoad<float, 2> {
+    static constexpr bool ok = true;
+    template <typename AT, typename P> __device__ static void load(P p, AT (&x)[2])
+    {
+        const v2f32 a = *vcast<v2f32>(p);
+        x[0] = (AT)a.x; x[1] = (AT)a.y;
+    }
+};
+template <> struct VecLoad<bf16_t, 4> {
+    static constexpr bool ok = true;
+    template <typename AT, typename P> __device__ static void load(P p, AT (&x)[4])
+    {
+        const v2u32 a = *vcast<v2u32>(p);
+        x[0] = (AT)__uint_as_float(a.x << 16); x[1] = (AT)__uint_as_float(a.x & 0xffff0000u);
+        x[2] = (AT)__uint_as_float(a.y << 16); x[3] = (AT)__uint_as_float(a.y & 0xffff0000u);
+    }
+};
+typedef uint32_t v4u32 __attribute__((ext_vector_type(4)));
+template <> struct VecLoad<bf16_t, 8> {
+    static constexpr bool ok = true;
+    template <typename AT, typename P> __device__ static void load(P p, AT (&x)[8])
+    {
+        const v4u32 a = *vcast<v4u32>(p);
+        x[0] = (AT)__uint_as_float(a.x << 16); x[1] = (AT)__uint_as_float(a.x & 0xffff0000u);
+        x[2] = (AT)__uint_as_float(a.y << 16); x[3] = (AT)__uint_as_float(a.y & 0xffff0000u);
+        x[4] = (AT)__uint_as_float(a.z << 16); x[5] = (AT)__uint_as_float(a.z & 0xffff0000u);
+        x[6] = (AT)__uint_as_float(a.w << 16); x[7] = (AT)__uint_as_float(a.w & 0xffff0000u);
+    }
+};
+template <> struct VecLoad<bf16_t, 2> {
+    static constexpr bool ok = true;
+    template <typename AT, typename P> __device__ static void load(P p, AT (&x)[2])
+    {
+        const uint32_t a = *vcast<uint32_t>(p);
+        x[0] = (AT)__uint_as_float(a << 16); x[1] = (AT)__uint_as_float(a & 0xffff0000u);
+    }
+};
+
+template <typename FT, typename AT, int G, int E, bool CONTIG = false, typename P>
+__device__ inline void load_cols(P row, int gl, int R, AT (&x)[E])
+{
+    if constexpr (CONTIG && VecLoad<FT, E>::ok) {
+        const int c0 = gl * E;
+        // bf16 rows of odd rank are only 2-byte aligned: element loads there
+        if (c0 + E <= R && (sizeof(FT) >= 4 || (R & 1) == 0)) {
+            VecLoad<FT, E>::load(row + c0, x);
+            return;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int c = col_of<CONTIG, G, E>(gl, q);
+        x[q] = c < R ? elem_acc<AT>(row, c, (FT*)nullptr) : (AT)0;
+    }
+}
+
+// factor row `off` (element offset) of other mode o: from the LDS copy of the
+// hot rows when it is there, else from global memory
+template <typename FT, typename AT, int G, int E, bool CONTIG, bool HOT>
+__device__ inline void load_row(const FT* __restrict__ U, lds_ptr<FT> hot, uint32_t hot_n, uint32_t off, int gl, int R,
+                                AT (&x)[E])
+{
+    if (HOT && off < hot_n) load_cols<FT, AT, G, E, CONTIG>(hot + off, gl, R, x);
+    else load_cols<FT, AT, G, E, CONTIG>(U + off, gl, R, x);
+}
+
+template <bool NT, typename T>
+__device__ inline T stream_ld(const T* p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+// Copies the leading hot_n[o] elements (the hot rows, in rank order) of each
+// other mode's rank-ordered factor into this workgroup's LDS, 16 bytes per
+// lane and step; every thread of the block takes part (called before any
+// thread returns).
+template <typename FT, typename VT, typename OT>
+__device__ inline void stage_hot_rows(const StreamArgs<FT, VT, OT>& a, unsigned char* smem)
+{
+    for (int o = 0; o < a.nother; ++o) {
+        const uint32_t words = (uint32_t)((a.hot_n[o] * sizeof(FT) + 15) / 16);
+        const uint4* __restrict__ src = reinterpret_cast<const uint4*>(a.Uo[o]);
+        uint4* dst = reinterpret_cast<uint4*>(smem + a.hot_base[o]);
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+}
+
+// HOT: the stream holds frequency ranks for the other modes and Uo are the
+// rank-ordered factor copies; the hot_n[o] / R most frequent rows of each are
+// read from LDS (staged once per workgroup), the rest from global memory.
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, bool NT, bool CONTIG, bool HOT>
+__global__ void __launch_bounds__(1024) k_mttkrp(StreamArgs<FT, VT, OT> a)
+{
+    constexpr int J = 4;  // nonzeros in flight per group
+    extern __shared__ __align__(16) unsigned char hot_smem[];
+    if constexpr (HOT) stage_hot_rows(a, hot_smem);
+    const int lane = threadIdx.x & 63;
+    const int gl = lane & (G - 1);
+    const uint64_t block = xcd_block(a.nblocks_real);
+    if (block == ~0ull) return;
+    const uint64_t group = (block * blockDim.x + threadIdx.x) / G;
+    const uint64_t s = group * a.seg;
+    if (s >= a.nnz) return;
+    const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
+    const int R = a.rank;
+    const int nother = a.nother;
+
+    AT acc[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) acc[q] = 0;
+    uint32_t cur = 0xffffffffu;
+
+    for (uint64_t p = s; p < e; p += G) {
+        // phase 0: lane gl decodes nonzero p+gl once (coalesced load)
+        uint32_t offs[ALTO_MAXN - 1];
+        uint32_t rown = 0;
+        VT myv = 0;
+        {
+            uint64_t k[PW];
+            if (p + gl < e) {
+#pragma unroll
+                for (int w = 0; w < PW; ++w) k[w] = stream_ld<NT>(&a.pk[(p + gl) * PW + w]);
+                myv = stream_ld<NT>(&a.val[p + gl]);
+            } else {
+#pragma unroll
+                for (int w = 0; w < PW; ++w) k[w] = 0;
+            }
+#pragma unroll
+            for (int o = 0; o < ALTO_MAXN - 1; ++o)
+                offs[o] = o < nother ? (uint32_t)(field<PW>(k, a.offo[o], a.bitso[o]) * (uint64_t)R) : 0u;
+            rown = (uint32_t)field<PW>(k, a.offn, a.bitsn);
+        }
+        const int cnt = (int)(e - p < (uint64_t)G ? e - p : (uint64_t)G);
+        for (int j0 = 0; j0 < cnt; j0 += J) {
+            // phase 1: broadcast J nonzeros, issue all their row gathers
+            AT prod[J][E];
+            uint32_t rj[J];
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+                rj[jj] = (uint32_t)__shfl((int)rown, j, G);
+                const AT v = (AT)shfl_val(myv, j, G);
+#pragma unroll
+                for (int q = 0; q < E; ++q) prod[jj][q] = v;
+            }
+#pragma unroll
+            for (int o = 0; o < ALTO_MAXN - 1; ++o) {
+                if (o < nother) {
+                    const FT* __restrict__ U = a.Uo[o];
+                    const lds_ptr<FT> H = (lds_ptr<FT>)(hot_smem + a.hot_base[o]);
+#pragma unroll
+                    for (int jj = 0; jj < J; ++jj) {
+                        const int j = j0 + jj < cnt ? j0 + jj : cnt - 1;
+                        const uint32_t off = (uint32_t)__shfl((int)offs[o], j, G);
+                        AT x[E];
+                        load_row<FT, AT, G, E, CONTIG, HOT>(U, H, a.hot_n[o], off, gl, R, x);
+#pragma unroll
+                        for (int q = 0; q < E; ++q) prod[jj][q] *= x[q];
+                    }
+                }
+            }
+            // phase 2: register reduction while the output row repeats
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                if (j0 + jj < cnt) {
+                    if (rj[jj] != cur) {
+                        if (cur != 0xffffffffu) flush_row<AT, OT, G, E, CONTIG>(a.out, cur, R, gl, acc);
+                        cur = rj[jj];
+#pragma unroll
+                        for (int q = 0; q < E; ++q) acc[q] = 0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < E; ++q) acc[q] += prod[jj][q];
+                }
+            }
+        }
+    }
+    if (cur != 0xffffffffu) flush_row<AT, OT, G, E, CONTIG>(a.out, cur, R, gl, acc);
+}
+
+// ------------------------------------------------ 16-byte lanes, VALU broadcasts
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// value of lane L (a constant) of each G-lane group, through VALU paths:
+// DPP quad_perm (G = 4), DPP row_newbcast (G = 8, 16: lane L of each 16-lane
+// row), readlane (G = 32, 64)
+template <int G, int L>
+__device__ __forceinline__ uint32_t bcast32(uint32_t v)
+{
+    static_assert(L >= 0 && L < G, "broadcast lane outside the group");
+    if constexpr (G == 64) {
+        return __builtin_amdgcn_readlane(v, L);
+    } else if constexpr (G == 32) {
+        const uint32_t lo = __builtin_amdgcn_readlane(v, L), hi = __builtin_amdgcn_readlane(v, 32 + L);
+        return (threadIdx.x & 32) ? hi : lo;
+    } else if constexpr (G == 16) {
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + L, 0xf, 0xf, false);
+    } else if constexpr (G == 8) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + L, 0xf, 0xf, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + 8 + L, 0xf, 0xf, false);
+        return (threadIdx.x & 8) ? hi : lo;
+    } else {
+        static_assert(G == 4, "group of 4..64 lanes");
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, L | (L << 2) | (L << 4) | (L << 6), 0xf, 0xf, false);
+    }
+}
+template <int G, int L> __device__ __forceinline__ double bcast_val(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint64_t r = ((uint64_t)bcast32<G, L>((uint32_t)(u >> 32)) << 32) | bcast32<G, L>((uint32_t)u);
+    return __longlong_as_double((long long)r);
+}
+template <int G, int L> __device__ __forceinline__ float bcast_val(float v)
+{
+    return __uint_as_float(bcast32<G, L>(__float_as_uint(v)));
+}
+
+// E = 16 / sizeof(FT) consecutive columns per lane in one 16-byte load; the
+// host guarantees 16-byte aligned rows (R * sizeof(FT) % 16 == 0), so a lane
+// either loads a whole vector or lies beyond the rank
+template <typename FT, typename AT, int E>
+__device__ __forceinline__ void load16(const FT* __restrict__ row, int gl, int R, AT (&x)[E])
+{
+    // branch-free: lanes beyond the rank read the row's first vector and
+    // drop it
+    const int c0 = gl * E;
+    const bool in = c0 < R;
+    VecLoad<FT, E>::load(row + (in ? c0 : 0), x);
+#pragma unroll
+    for (int q = 0; q < E; ++q) x[q] = in ? x[q] : (AT)0;
+}
+
+// The MTTKRP stream kernel with 16-byte lanes: per batch of G nonzeros each
+// lane decodes one, then the group walks them J at a time; the decoded
+// fields reach the group through VALU broadcasts (no LDS permutes) and each
+// factor row arrives with one 16-byte load per lane.  The next batch's
+// nonzeros are loaded while the current batch's rows are gathered.
+// NO = number of other modes (a template constant, so that the gathers of
+// all modes are issued back to back with no branch between them).
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO>
+__global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
+{
+    constexpr int J = G < 4 ? G : 4;  // nonzeros in flight per group
+    const int lane = threadIdx.x & 63;
+    const int gl = lane & (G - 1);
+    const uint64_t block = xcd_block(a.nblocks_real);
+    if (block == ~0ull) return;
+    const uint64_t group = (block * 256 + threadIdx.x) / G;
+    const uint64_t s = group * a.seg;
+    if (s >= a.nnz) return;
+    const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
+    const int R = a.rank;
+
+    AT acc[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) acc[q] = 0;
+    uint32_t cur = 0xffffffffu;
+
+    // lane gl's nonzero of the next batch (past the segment: row 0 of every
+    // mode, value 0 -- valid memory, and never accumulated)
+    uint64_t kn[PW];
+    VT vn = 0;
+#pragma unroll
+    for (int w = 0; w < PW; ++w) kn[w] = 0;
+    if (s + gl < e) {
+#pragma unroll
+        for (int w = 0; w < PW; ++w) kn[w] = a.pk[(s + gl) * PW + w];
+        vn = a.val[s + gl];
+    }
+
+    for (uint64_t p = s; p < e; p += G) {
+        uint32_t offs[NO > 0 ? NO : 1];
+        uint32_t rown;
+        const VT myv = vn;
+#pragma unroll
+        for (int o = 0; o < NO; ++o) offs[o] = (uint32_t)(field<PW>(kn, a.offo[o], a.bitso[o]) * (uint64_t)R);
+        rown = (uint32_t)field<PW>(kn, a.offn, a.bitsn);
+        const int cnt = (int)(e - p < (uint64_t)G ? e - p : (uint64_t)G);
+        static_for<0, G / J>([&](auto bi) {
+            constexpr int j0 = decltype(bi)::value * J;
+            if (j0 >= cnt) return;
+            AT prod[J][E];
+            uint32_t rj[J];
+            static_for<0, J>([&](auto ji) {
+                constexpr int jj = decltype(ji)::value;
+                rj[jj] = bcast32<G, j0 + jj>(rown);
+                const AT v = (AT)bcast_val<G, j0 + jj>(myv);
+#pragma unroll
+                for (int q = 0; q < E; ++q) prod[jj][q] = v;
+            });
+            static_for<0, NO>([&](auto oi) {
+                constexpr int o = decltype(oi)::value;
+                const FT* __restrict__ U = a.Uo[o];
+                static_for<0, J>([&](auto ji) {
+                    constexpr int jj = decltype(ji)::value;
+                    const uint32_t off = bcast32<G, j0 + jj>(offs[o]);
+                    AT x[E];
+                    load16<FT, AT, E>(U + off, gl, R, x);
+#pragma unroll
+                    for (int q = 0; q < E; ++q) prod[jj][q] *= x[q];
+                });
+            });
+            if constexpr (j0 == 0) {
+                // the next batch's nonzeros, issued behind this block's gathers
+                const uint64_t pn = p + G + gl;
+#pragma unroll
+                for (int w = 0; w < PW; ++w) kn[w] = 0;
+                vn = 0;
+                if (pn < e) {
+#pragma unroll
+                    for (int w = 0; w < PW; ++w) kn[w] = a.pk[pn * PW + w];
+                    vn = a.val[pn];
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                if (j0 + jj < cnt) {
+                    if (rj[jj] != cur) {
+                        if (cur != 0xffffffffu) flush_row<AT, OT, G, E, true>(a.out, cur, R, gl, acc);
+                        cur = rj[jj];
+#pragma unroll
+                        for (int q = 0; q < E; ++q) acc[q] = 0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < E; ++q) acc[q] += prod[jj][q];
+                }
+            }
+        });
+    }
+    if (cur != 0xffffffffu) flush_row<AT, OT, G, E, true>(a.out, cur, R, gl, acc);
+}
+
+// the 16-byte-lane kernel needs 16-byte aligned factor rows
+template <typename FT, typename VT, typename OT>
+inline bool rows_16b_aligned(const StreamArgs<FT, VT, OT>& a, int R)
+{
+    if (((uint64_t)R * sizeof(FT)) % 16 != 0) return false;
+    for (int o = 0; o < a.nother; ++o)
+        if ((uintptr_t)a.Uo[o] % 16 != 0) return false;
+    return true;
+}
+
+// Lane mappings: 0 = 2 strided columns per lane (each gather instruction
+// reads G consecutive elements of a row); 1 = 4 contiguous columns per lane;
+// 2 = E = 16 / sizeof(FT) contiguous columns per lane in one 16-byte load,
+// broadcasts through DPP / readlane.  -1 picks per element type, from
+// measurements on MI355X at rank 32: FP64 and FP32 -> 2, BF16 -> 1.
+// Ranked streams (hot rows in LDS) use mapping 0 or 1.
+template <typename FT, typename VT, typename AT, typename OT>
+int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, const VT* val, OT* out, int seg,
+                  const MttkrpTune& k, hipStream_t s)
+{
+    StreamArgs<FT, VT, OT> a;
+    fill_stream_args(t, mode, R, seg, a);
+    ALTO_TRY(set_factors(t, mode, U, a, s));
+    a.val = val;
+    a.out = out;
+    int v = k.variant >= 0 ? k.variant : std::is_same<FT, bf16_t>::value ? 1 : 2;
+    const bool hot = t->ms[mode].ranked != 0 && a.nother > 0;
+    constexpr int E16 = 16 / sizeof(FT);
+    if (v == 2 && (hot || !rows_16b_aligned(a, R) || R > 64 * E16 || a.nother < 1)) v = 0;
+    if (v == 2) {
+        int G = 4;
+        while (G * E16 < R) G <<= 1;
+        dim3 grid = stream_grid(t->nnz, seg, G, a);
+#define K16N_(GG, PWW)                                                                                          \
+    do {                                                                                                        \
+        switch (a.nother) {                                                                                     \
+        case 1: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 1>), grid, dim3(256), 0, s, a); break; \
+        case 2: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 2>), grid, dim3(256), 0, s, a); break; \
+        case 3: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 3>), grid, dim3(256), 0, s, a); break; \
+        default: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 4>), grid, dim3(256), 0, s, a); break; \
+        }                                                                                                       \
+    } while (0)
+#define K16_(GG)                                 \
+    do {                                         \
+        if (G == GG) {                           \
+            if (t->pack.pw == 1) K16N_(GG, 1);   \
+            else K16N_(GG, 2);                   \
+        }                                        \
+    } while (0)
+        K16_(4);
+        K16_(8);
+        K16_(16);
+        K16_(32);
+        K16_(64);
+#undef K16_
+#undef K16N_
+        HIP_TRY(hipGetLastError());
+        return ALTO_OK;
+    }
+    const int E = v == 1 ? 4 : 2;
+    int G = 4;
+    while (G * E < R && G < 64) G <<= 1;
+    const int bt = hot ? k.hot_block : 256;
+    const uint32_t lds = hot ? plan_hot_rows(t, mode, a, (uint32_t)k.hot_lds) : 0;
+    dim3 grid = stream_grid(t->nnz, seg, G, a, bt);
+#define K_(PWW, NTT, HOTT) \
+    hipLaunchKernelGGL((k_mttkrp<FT, VT, AT, OT, GG, EE, PWW, NTT, CC, HOTT>), grid, dim3(bt), lds, s, a)
+#define L_(GG_, EE_, CC_)                            \
+    do {                                             \
+        constexpr int GG = GG_, EE = EE_;            \
+        constexpr bool CC = CC_;                     \
+        if (t->pack.pw == 1) {                       \
+            if (hot) K_(1, false, true);             \
+            else if (k.nt) K_(1, true, false);       \
+            else K_(1, false, false);                \
+        } else {                                     \
+            if (hot) K_(2, false, true);             \
+            else K_(2, false, false);                \
+        }                                            \
+    } while (0)
+#define LG_(EE, CC)                         \
+    do {                                    \
+        switch (G) {                        \
+        case 4: L_(4, EE, CC); break;       \
+        case 8: L_(8, EE, CC); break;       \
+        case 16: L_(16, EE, CC); break;     \
+        case 32: L_(32, EE, CC); break;     \
+        default: L_(64, EE, CC); break;     \
+        }                                   \
+    } while (0)
+    if (v == 1) LG_(4, true);
+    else LG_(2, false);
+#undef LG_
+#undef L_
+#undef K_
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+int launch_mttkrp_f64(const alto_tensor* t, int mode, int R, const double* const* U, const double* val, double* out,
+                      int seg, const MttkrpTune& k, hipStream_t s);
+int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const* U, const float* val, float* out,
+                      int seg, const MttkrpTune& k, hipStream_t s);
+int launch_mttkrp_bf16(const alto_tensor* t, int mode, int R, const bf16_t* const* U, const float* val, float* out,
+                       int seg, const MttkrpTune& k, hipStream_t s);
+
+}  // namespace alto

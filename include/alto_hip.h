@@ -112,17 +112,20 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * log2 of the ALTO chunk inside which each mode's stream is row-ordered
  * (0 = default); streams are re-ordered on their next use.
  * ALTO_TUNE_NONTEMPORAL: 1 = stream the nonzeros with non-temporal loads.
- * ALTO_TUNE_MTTKRP_VARIANT: lane mapping of the MTTKRP kernel, 0..3 (2 strided
- * or 4 contiguous columns per lane; 4 or 8 nonzeros in flight per group), -1
- * (default) = chosen per factor element type.
+ * ALTO_TUNE_MTTKRP_VARIANT: lane mapping of the MTTKRP kernel: 0 = 2 strided
+ * columns per lane, 1 = 4 contiguous columns per lane, 2 = one 16-byte load
+ * per lane and row (needs rank * element size % 16 == 0, else 0 is used);
+ * -1 (default) = chosen per factor element type.
  * ALTO_TUNE_GRAPHS: 1 = CP-ALS sessions capture one iteration as a HIP graph
  * and replay it (non-verbose iterate calls; phase timers are then not kept).
- * ALTO_TUNE_FIBER_ORDER: 1 (default) = inside one output row the stream is
- * ordered by the row of the other mode with the smallest dimension, and the
- * kernels reuse that factor row while it repeats; 0 = ALTO order inside the
- * row.  Streams are re-ordered on their next use. */
+ * ALTO_TUNE_HOT_ROWS: 1 = the streams address the other modes' factor rows
+ * by frequency rank, and each MTTKRP workgroup stages the most frequent rows
+ * of each input factor in LDS (strided lane mapping only); 0 (default) = rows
+ * from global memory.  Streams are re-ordered on their next use.
+ * ALTO_TUNE_HOT_BLOCK: threads per workgroup of that kernel (256/512/1024).
+ * ALTO_TUNE_HOT_LDS: LDS bytes per workgroup for the staged rows. */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
-       ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_FIBER_ORDER = 6 };
+       ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9 };
 int alto_tune_set(int param, int value);
 
 /* ---- CP-ALS --------------------------------------------------------------- */

@@ -144,26 +144,32 @@ def test_mttkrp_accumulate_and_zipf_heavy_rows():
 def tune_reset():
     A = _alto()
     yield A
-    for param, value in [(6, 1), (4, -1), (3, 0), (2, 0), (1, 0)]:
+    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 49152)]:
         A.check(A.lib().alto_tune_set(param, value), "tune_set")
 
 
-@pytest.mark.parametrize("fiber", [0, 1])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("hot", [(0, 512, 49152), (1, 512, 49152), (1, 1024, 0), (1, 256, 20000)],
+                         ids=lambda h: f"hot{h[0]}_bt{h[1]}_lds{h[2]}")
+@pytest.mark.parametrize("variant", [-1, 0, 1, 2])
 @pytest.mark.parametrize("chunk_log2", [0, 10])
-def test_mttkrp_tuning_knobs_keep_results(tune_reset, fiber, variant, chunk_log2):
-    """Stream order inside a row (fiber order on/off), the lane mapping and
-    the chunk size change only the summation order: every combination matches
-    the oracle in FP64, FP32 and BF16, with the tolerances of the tests above."""
+def test_mttkrp_tuning_knobs_keep_results(tune_reset, hot, variant, chunk_log2):
+    """Rank-addressed streams with hot rows staged in LDS (on/off, workgroup
+    size, LDS budget including none), the lane mapping and the chunk size
+    change only the summation order: every combination matches the oracle in
+    FP64, FP32 and BF16, with the tolerances of the tests above.  Rank 7 and
+    100 rows are not 16-byte aligned in every precision (the 16-byte mapping
+    then falls back)."""
     A = tune_reset
-    A.check(A.lib().alto_tune_set(6, fiber), "tune_set")
+    A.check(A.lib().alto_tune_set(7, hot[0]), "tune_set")
+    A.check(A.lib().alto_tune_set(8, hot[1]), "tune_set")
+    A.check(A.lib().alto_tune_set(9, hot[2]), "tune_set")
     A.check(A.lib().alto_tune_set(4, variant), "tune_set")
     A.check(A.lib().alto_tune_set(2, chunk_log2), "tune_set")
     dims = [300, 40, 700, 25]
     cidx, vals = zipf_tensor(dims, 30000, 17)
     g = A.AltoTensor(dims, cidx, vals, nprtn=8)
     o = oracle.OracleAlto(dims, cidx, vals, W=1, nprtn=8)
-    for R in (7, 32, 100):
+    for R in (7, 8, 32, 64, 100):
         rng = np.random.default_rng(R)
         U = [rng.random((d, R)) for d in dims]
         for mode in range(len(dims)):
