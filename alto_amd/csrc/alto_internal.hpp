@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "alto_hip.h"
@@ -48,6 +49,13 @@ struct DevBuf {
     ~DevBuf() { release(); }
     int alloc(size_t n);
     void release();
+    void swap(DevBuf& o)
+    {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+        std::swap(cap, o.cap);
+        std::swap(device, o.device);
+    }
     template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
@@ -80,6 +88,13 @@ struct ModeStream {
     // row with the most nonzeros) instead of rows; the kernels then read
     // the other factors through row copies permuted into rank order
     int ranked = 0;
+    // Hot-pattern split of a ranked stream: the nonzeros are grouped by their
+    // pattern (bit o set when the rank of the o-th other mode is below
+    // hot_k[o]), then by chunk and row; pattern p occupies
+    // [pat_start[p], pat_start[p+1]).  split = 0: not grouped.
+    int split = 0;
+    int hot_k[ALTO_MAXN - 1] = {0};
+    std::vector<uint64_t> pat_start;
 };
 
 }  // namespace alto
@@ -133,5 +148,7 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s);
 int tuned_chunk_log2();
 int hot_rows_enabled();
 int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStream_t s);
+int split_stream(alto_tensor* t, int mode, const int* hot_k, hipStream_t s);
+int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s);
 
 }  // namespace alto

@@ -294,8 +294,9 @@ struct CpdSession : CpdSessionBase {
         prec = precision;
         s = st;
         c.prec = prec;
-        ALTO_TRY(prepare_mode_streams(t, prec == ALTO_PREC_F64, prec != ALTO_PREC_F64, s));
-        // reserved here, outside any graph capture of the iterations
+        // stream layout and rank-ordered factor buffers are set up here,
+        // outside any graph capture of the iterations
+        ALTO_TRY(mttkrp_prepare(t, R, prec, s));
         ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, 8, s));
         ALTO_TRY(cpd_alloc(t, R, c));
         ALTO_TRY(cpd_upload(t, c, Uh, s, dev_seed));

@@ -871,6 +871,111 @@ int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStr
     return ALTO_OK;
 }
 
+// ------------------------------------------------------- hot-pattern split
+struct HotK {
+    int k[ALTO_MAXN - 1];
+};
+
+// sort key of the split: (pattern, chunk, row); the stream is ranked, so the
+// other modes' fields are ranks
+template <int PW>
+__global__ void __launch_bounds__(256) k_split_key(const uint64_t* __restrict__ pk, uint64_t n, PackLayout L, int mode,
+                                                   HotK hk, int chunk_log2, int cbits, uint64_t* __restrict__ ck,
+                                                   uint32_t* __restrict__ pay, unsigned int* __restrict__ pat_count)
+{
+    __shared__ unsigned int s_cnt[16];
+    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k[PW];
+#pragma unroll
+        for (int w = 0; w < PW; ++w) k[w] = pk[i * PW + w];
+        uint64_t pat = 0;
+        int o = 0;
+        for (int m = 0; m < L.nmode; ++m) {
+            if (m == mode) continue;
+            if (field<PW>(k, L.off[m], L.bits[m]) < (uint64_t)hk.k[o]) pat |= 1ull << o;
+            ++o;
+        }
+        const uint64_t r = field<PW>(k, L.off[mode], L.bits[mode]);
+        ck[i] = (((pat << cbits) | (i >> chunk_log2)) << L.bits[mode]) | r;
+        pay[i] = (uint32_t)i;
+        atomicAdd(&s_cnt[pat], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16 && s_cnt[threadIdx.x]) atomicAdd(&pat_count[threadIdx.x], s_cnt[threadIdx.x]);
+}
+
+template <int PW>
+__global__ void __launch_bounds__(256) k_split_gather(const uint64_t* __restrict__ pk, const double* __restrict__ v64,
+                                                      const float* __restrict__ v32, const uint32_t* __restrict__ perm,
+                                                      uint64_t n, uint64_t* __restrict__ pk_out,
+                                                      double* __restrict__ v64_out, float* __restrict__ v32_out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t src = perm[i];
+#pragma unroll
+        for (int w = 0; w < PW; ++w) pk_out[i * PW + w] = pk[src * PW + w];
+        if (v64) v64_out[i] = v64[src];
+        if (v32) v32_out[i] = v32[src];
+    }
+}
+
+// Groups a ranked stream by hot pattern (see ModeStream); hot_k[o] = rows of
+// the o-th other mode counted as hot.  Synchronous on `s`.
+int split_stream(alto_tensor* t, int mode, const int* hot_k, hipStream_t s)
+{
+    ModeStream& M = t->ms[mode];
+    if (!M.ranked) {
+        set_error("split_stream: stream %d is not ranked", mode);
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    const int no = t->nmode - 1;
+    const uint64_t n = t->nnz;
+    const int npat = 1 << no;  // at most 16 (five modes)
+    uint64_t nchunks = (n + (1ull << M.chunk_log2) - 1) >> M.chunk_log2;
+    int cbits = 0;
+    while ((1ull << cbits) < nchunks) ++cbits;
+    const int bits = no + cbits + t->pack.bits[mode];
+    if (bits > 64) {
+        set_error("split_stream: sort key of %d bits", bits);
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    HotK hk;
+    for (int o = 0; o < ALTO_MAXN - 1; ++o) hk.k[o] = o < no ? hot_k[o] : 0;
+    DevBuf ck, pay, cnt, pk2, v64, v32;
+    ALTO_TRY(ck.alloc(n * 8));
+    ALTO_TRY(pay.alloc(n * 4));
+    ALTO_TRY(cnt.alloc(64 * 4));
+    ALTO_TRY(fill_zero(cnt.as<uint32_t>(), 64, s));
+    ALTO_TRY(pk2.alloc(n * t->pack.pw * 8));
+    if (M.val64.p) ALTO_TRY(v64.alloc(n * 8));
+    if (M.val32.p) ALTO_TRY(v32.alloc(n * 4));
+    RANK_DISPATCH_PW(t->pack.pw, {
+        hipLaunchKernelGGL((k_split_key<PW>), dim3(grid_for(n)), dim3(256), 0, s, M.pk.as<uint64_t>(), n, t->pack, mode,
+                           hk, M.chunk_log2, cbits, ck.as<uint64_t>(), pay.as<uint32_t>(), cnt.as<unsigned int>());
+    });
+    HIP_TRY(hipGetLastError());
+    ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, bits, s));
+    RANK_DISPATCH_PW(t->pack.pw, {
+        hipLaunchKernelGGL((k_split_gather<PW>), dim3(grid_for(n)), dim3(256), 0, s, M.pk.as<uint64_t>(),
+                           M.val64.as<double>(), M.val32.as<float>(), pay.as<uint32_t>(), n, pk2.as<uint64_t>(),
+                           v64.as<double>(), v32.as<float>());
+    });
+    HIP_TRY(hipGetLastError());
+    uint32_t hc[64];
+    HIP_TRY(hipMemcpyAsync(hc, cnt.p, sizeof(hc), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    M.pk.swap(pk2);
+    if (M.val64.p) M.val64.swap(v64);
+    if (M.val32.p) M.val32.swap(v32);
+    M.pat_start.assign(npat + 1, 0);
+    for (int p = 0; p < npat; ++p) M.pat_start[p + 1] = M.pat_start[p] + hc[p];
+    for (int o = 0; o < ALTO_MAXN - 1; ++o) M.hot_k[o] = hk.k[o];
+    M.split = 1;
+    return ALTO_OK;
+}
+
 // Ranked streams need every mode's rows below 2^32 (32-bit ranks) and a sort
 // key of 32 count bits + row bits.
 static bool rank_applicable(const alto_tensor* t)
@@ -906,6 +1011,8 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
             M.val64.release();
             M.val32.release();
             M.ranked = 0;
+            M.split = 0;
+            M.pat_start.clear();
         }
     }
     DevBuf ck, pay;
@@ -916,11 +1023,14 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         if (M.pk.p && (!need64 || M.val64.p) && (!need32 || M.val32.p)) continue;
         M.chunk_log2 = t->chunk_log2;
         M.ranked = 0;
-        // an existing stream has this order already (the sort is deterministic):
-        // rewriting it leaves its other value array valid
+        M.split = 0;
+        M.pat_start.clear();
+        // the stream is rewritten in its base order: every value array it
+        // has is rewritten with it
+        const bool w64 = need64 || M.val64.p, w32 = need32 || M.val32.p;
         if (!M.pk.p) ALTO_TRY(M.pk.alloc(n * t->pack.pw * 8));
-        if (need64 && !M.val64.p) ALTO_TRY(M.val64.alloc(n * 8));
-        if (need32 && !M.val32.p) ALTO_TRY(M.val32.alloc(n * 4));
+        if (w64 && !M.val64.p) ALTO_TRY(M.val64.alloc(n * 8));
+        if (w32 && !M.val32.p) ALTO_TRY(M.val32.alloc(n * 4));
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s, t->key.as<uint64_t>(),
                                lutref(t->delin), t->pack, m, t->chunk_log2, n, ck.as<uint64_t>(),
@@ -931,8 +1041,8 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_gather<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
                                t->key.as<uint64_t>(), t->vals.as<double>(), lutref(t->delin),
-                               pay.as<uint32_t>(), n, M.pk.as<uint64_t>(), need64 ? M.val64.as<double>() : nullptr,
-                               need32 ? M.val32.as<float>() : nullptr);
+                               pay.as<uint32_t>(), n, M.pk.as<uint64_t>(), w64 ? M.val64.as<double>() : nullptr,
+                               w32 ? M.val32.as<float>() : nullptr);
         });
         HIP_TRY(hipGetLastError());
     }

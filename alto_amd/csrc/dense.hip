@@ -8,8 +8,9 @@
 //    in one workgroup (FP64); when Cholesky fails, the minimum-norm
 //    pseudo-inverse from a Jacobi eigen-decomposition.
 //    Reference: pseudo_inverse (include/cpd.hpp:546-646).
-//  - Row solves U_n[i,:] = M[i,:] V^-1 (two triangular solves per row, one
-//    lane group per row), fused with the per-block column norms.
+//  - Row solves U_n[i,:] = M[i,:] V^-1 as a GEMM with the explicit inverse
+//    V^-1 = L^-T L^-1 from the Cholesky factor (or the pseudo-inverse),
+//    MFMA FP64, fused with the per-block column norms.
 //  - Column normalisation (2-norm / max-norm), the fit, ||X||^2.
 //    Reference: KruskalModelNorm (src/kruskal_model.cpp:224-341),
 //    cpd_fit_alto (include/cpd.hpp:189-271).
@@ -123,16 +124,16 @@ template int gram_device<bf16_t>(const bf16_t*, uint64_t, int, double*, double*,
 
 // ------------------------------------------------------------------ V and its Cholesky factor
 // V = prod_{m != mode} G_m (elementwise; all ones for a 1-mode tensor).
-// Right-looking Cholesky of the lower triangle in LDS.  status: 0 =
-// Cholesky (L in Lout), 1 = pseudo-inverse (Pout).
-template <bool LDS>
+// Right-looking Cholesky of the lower triangle in LDS (R <= 128, one
+// workgroup of 256 threads; the trailing update runs on a 32 x 8 thread
+// grid: columns on x, rows on y).  status: 0 = Cholesky (L in Lout), 1 =
+// not positive definite (the pseudo-inverse path takes over).
 __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ grams, int nmode, int mode, int R,
-                                              double* __restrict__ ws, double* __restrict__ Lout, int* __restrict__ status)
+                                              double* __restrict__ Lout, int* __restrict__ status)
 {
-    extern __shared__ double s_A[];
-    double* A = LDS ? s_A : ws;
+    extern __shared__ double A[];
     __shared__ int ok;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
     for (int k = tid; k < R * R; k += blockDim.x) {
         double v = 1.0;
         for (int m = 0; m < nmode; ++m)
@@ -152,10 +153,9 @@ __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ grams, 
         const double djj = A[j * R + j];
         for (int i = j + 1 + tid; i < R; i += blockDim.x) A[i * R + j] /= djj;
         __syncthreads();
-        const int w = R - 1 - j;  // trailing size
-        for (int idx = tid; idx < w * w; idx += blockDim.x) {
-            const int i = j + 1 + idx / w, k = j + 1 + idx % w;
-            if (k <= i) A[i * R + k] -= A[i * R + j] * A[k * R + j];
+        for (int i = j + 1 + ty; i < R; i += 8) {
+            const double aij = A[i * R + j];
+            for (int k = j + 1 + tx; k <= i; k += 32) A[i * R + k] -= aij * A[k * R + j];
         }
         __syncthreads();
     }
@@ -166,6 +166,34 @@ __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ grams, 
         }
     }
     if (tid == 0) *status = ok ? 0 : 1;
+}
+
+// V^-1 = L^-T L^-1 from the Cholesky factor (status 0) into W; with status 1
+// W already holds the pseudo-inverse.  One workgroup: the columns of L^-1 by
+// forward substitution (a thread per column), then the product.
+__global__ void __launch_bounds__(256) k_chol_inverse(const double* __restrict__ L, int R, const int* __restrict__ status,
+                                                      double* __restrict__ W)
+{
+    if (*status != 0) return;
+    extern __shared__ double Li[];  // L^-1, row stride R + 1 (L itself is read from global memory)
+    const int LS = R + 1;
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) Li[(k / R) * LS + k % R] = 0.0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < R; j += blockDim.x) {
+        Li[j * LS + j] = 1.0 / L[j * R + j];
+        for (int i = j + 1; i < R; ++i) {
+            double acc = 0;
+            for (int k = j; k < i; ++k) acc += L[i * R + k] * Li[k * LS + j];
+            Li[i * LS + j] = -acc / L[i * R + i];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
+        const int a = e / R, b = e % R;
+        double acc = 0;
+        for (int k = a > b ? a : b; k < R; ++k) acc += Li[k * LS + a] * Li[k * LS + b];
+        W[e] = acc;
+    }
 }
 
 // Minimum-norm pseudo-inverse V^+ = Q diag(1/e) Q^T over |e| > R*eps*max|e|
@@ -275,105 +303,89 @@ __global__ void __launch_bounds__(256) k_pinv(const double* __restrict__ grams, 
 }
 
 // ------------------------------------------------------------------ row solves
-// One lane group per row, column c = gl + q*G held by lane gl.  Cholesky path:
-// forward then backward substitution (POTRS); fallback: x = V^+ u.  Writes
-// the FP64/FP32 master row and the per-block column sums of squares / max.
-template <typename MT, typename UT, int G, int E>
-__global__ void __launch_bounds__(256) k_solve(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ L,
-                                               const double* __restrict__ P, const int* __restrict__ status,
+// U[i,:] = M[i,:] W with W = V^-1 (or V^+), as a GEMM on the matrix cores
+// (v_mfma_f64_16x16x4f64): each wave takes 16 rows at a time; per k-step of
+// 4 it loads one M element per lane (A operand: lane l holds M[row0 + l%16]
+// [k0 + l/16]) and multiplies it into every 16-column tile of the output
+// (B operand from the LDS copy of W: lane l holds W[k0 + l/16][c0 + l%16]).
+// Fused with the per-block column sums of squares / maxima of the stored
+// values (for lambda).
+constexpr int APPLY_MAXT = 8;  // 16-column tiles (R <= 128)
+
+template <typename MT, typename UT>
+__global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ Wg,
                                                UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax)
 {
-    const int gl = threadIdx.x & (G - 1);
-    const int gid = threadIdx.x / G;
-    constexpr int GPB = 256 / G;
-    const bool chol = *status == 0;
-    double ss[E], mx[E];
+    extern __shared__ double sW[];  // W, row stride WS
+    const int WS = R + 2;
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) sW[(k / R) * WS + k % R] = Wg[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int T16 = (R + 15) / 16, KS = (R + 3) / 4;
+    double ss[APPLY_MAXT], mx[APPLY_MAXT];
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-        ss[q] = 0;
-        mx[q] = -INFINITY;
+    for (int t = 0; t < APPLY_MAXT; ++t) {
+        ss[t] = 0;
+        mx[t] = -INFINITY;
     }
-    for (uint64_t i = (uint64_t)blockIdx.x * GPB + gid; i < I; i += (uint64_t)gridDim.x * GPB) {
-        double t[E];
+    for (uint64_t row0 = ((uint64_t)blockIdx.x * 4 + wave) * 16; row0 < I; row0 += (uint64_t)gridDim.x * 64) {
+        d4_t acc[APPLY_MAXT];
 #pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int c = gl + q * G;
-            t[q] = c < R ? (double)M[i * R + c] : 0.0;
+        for (int t = 0; t < APPLY_MAXT; ++t) acc[t] = d4_t{0, 0, 0, 0};
+        const uint64_t ra = row0 + r16;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k = ks * 4 + kq;
+            const double a = (ra < I && k < R) ? (double)M[ra * R + k] : 0.0;
+#pragma unroll
+            for (int t = 0; t < APPLY_MAXT; ++t) {
+                if (t < T16) {
+                    const int c = t * 16 + r16;
+                    const double b = (k < R && c < R) ? sW[k * WS + c] : 0.0;
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                }
+            }
         }
-        if (chol) {
-            for (int k = 0; k < R; ++k) {  // L y = u
-                const int owner = k % G, qk = k / G;
-                double tk = 0;
+        // register v of lane l holds C[l/16 + 4v][l%16]
 #pragma unroll
-                for (int q = 0; q < E; ++q)
-                    if (q == qk) tk = t[q];
-                const double yk = __shfl(tk, owner, G) / L[k * R + k];
+        for (int t = 0; t < APPLY_MAXT; ++t) {
+            if (t < T16) {
+                const int c = t * 16 + r16;
 #pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    const int c = gl + q * G;
-                    if (c == k) t[q] = yk;
-                    else if (c > k && c < R) t[q] -= L[c * R + k] * yk;
+                for (int v = 0; v < 4; ++v) {
+                    const uint64_t r = row0 + kq + 4 * v;
+                    if (r < I && c < R) {
+                        const UT x = (UT)acc[t][v];
+                        U[r * R + c] = x;
+                        const double xd = (double)x;
+                        ss[t] += xd * xd;
+                        mx[t] = fmax(mx[t], xd);
+                    }
                 }
-            }
-            for (int k = R - 1; k >= 0; --k) {  // L^T x = y
-                const int owner = k % G, qk = k / G;
-                double tk = 0;
-#pragma unroll
-                for (int q = 0; q < E; ++q)
-                    if (q == qk) tk = t[q];
-                const double xk = __shfl(tk, owner, G) / L[k * R + k];
-#pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    const int c = gl + q * G;
-                    if (c == k) t[q] = xk;
-                    else if (c < k) t[q] -= L[k * R + c] * xk;
-                }
-            }
-        } else {
-            double x[E];
-#pragma unroll
-            for (int q = 0; q < E; ++q) x[q] = 0;
-            for (int k = 0; k < R; ++k) {
-                const int owner = k % G, qk = k / G;
-                double tk = 0;
-#pragma unroll
-                for (int q = 0; q < E; ++q)
-                    if (q == qk) tk = t[q];
-                const double uk = __shfl(tk, owner, G);
-#pragma unroll
-                for (int q = 0; q < E; ++q) {
-                    const int c = gl + q * G;
-                    if (c < R) x[q] += P[c * R + k] * uk;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < E; ++q) t[q] = x[q];
-        }
-#pragma unroll
-        for (int q = 0; q < E; ++q) {
-            const int c = gl + q * G;
-            if (c < R) {
-                U[i * R + c] = (UT)t[q];
-                const double v = (double)(UT)t[q];
-                ss[q] += v * v;
-                mx[q] = fmax(mx[q], v);
             }
         }
     }
-    // block reduction of the column statistics: groups combine through LDS
-    __shared__ double s_ss[256 * E], s_mx[256 * E];
+    // column statistics: lanes with equal l%16 hold the same column
+    __shared__ double s_ss[4 * 16 * APPLY_MAXT], s_mx[4 * 16 * APPLY_MAXT];
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-        s_ss[threadIdx.x * E + q] = ss[q];
-        s_mx[threadIdx.x * E + q] = mx[q];
+    for (int t = 0; t < APPLY_MAXT; ++t) {
+        double a = ss[t], b = mx[t];
+        a += __shfl_xor(a, 16);
+        b = fmax(b, __shfl_xor(b, 16));
+        a += __shfl_xor(a, 32);
+        b = fmax(b, __shfl_xor(b, 32));
+        if (kq == 0) {
+            s_ss[(wave * APPLY_MAXT + t) * 16 + r16] = a;
+            s_mx[(wave * APPLY_MAXT + t) * 16 + r16] = b;
+        }
     }
     __syncthreads();
     for (int c = threadIdx.x; c < R; c += blockDim.x) {
-        const int l = c % G, q = c / G;
+        const int t = c / 16, l = c % 16;
         double a = 0, b = -INFINITY;
-        for (int g = 0; g < GPB; ++g) {
-            a += s_ss[(g * G + l) * E + q];
-            b = fmax(b, s_mx[(g * G + l) * E + q]);
+        for (int w = 0; w < 4; ++w) {
+            a += s_ss[(w * APPLY_MAXT + t) * 16 + l];
+            b = fmax(b, s_mx[(w * APPLY_MAXT + t) * 16 + l]);
         }
         psum[(size_t)blockIdx.x * R + c] = a;
         pmax[(size_t)blockIdx.x * R + c] = b;
@@ -519,10 +531,9 @@ int random_factor_device(double* U, uint64_t n, uint64_t seed, int mode, hipStre
 // ------------------------------------------------------------------ host wrappers
 int solve_blocks(uint64_t I, int R)
 {
-    int G = group_lanes(R);
-    uint64_t gpb = 256 / G;
-    uint64_t b = (I + gpb - 1) / gpb;
-    if (b > 1024) b = 1024;
+    (void)R;
+    uint64_t b = (I + 63) / 64;  // 64 rows per workgroup pass
+    if (b > 2048) b = 2048;
     return b < 1 ? 1 : (int)b;
 }
 
@@ -533,11 +544,9 @@ int chol_device(const double* grams, int nmode, int mode, int R, double* L, doub
         set_error("pseudo-inverse: rank above 128");
         return ALTO_ERR_UNSUPPORTED;
     }
-    if (R <= 64)
-        hipLaunchKernelGGL((k_chol<true>), dim3(1), dim3(64), (size_t)R * R * 8, s, grams, nmode, mode, R, wsA, L, status);
-    else
-        hipLaunchKernelGGL((k_chol<false>), dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, L, status);
+    hipLaunchKernelGGL(k_chol, dim3(1), dim3(256), (size_t)R * R * 8, s, grams, nmode, mode, R, L, status);
     hipLaunchKernelGGL(k_pinv, dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, wsQ, P, status);
+    hipLaunchKernelGGL(k_chol_inverse, dim3(1), dim3(256), (size_t)R * (R + 1) * 8, s, L, R, status, P);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }
@@ -546,10 +555,9 @@ template <typename MT, typename UT>
 int solve_device(const MT* M, uint64_t I, int R, const double* L, const double* P, const int* status, UT* U,
                  double* psum, double* pmax, int nb, hipStream_t s)
 {
-    RANK_DISPATCH(R, 1, {
-        (void)PW;
-        hipLaunchKernelGGL((k_solve<MT, UT, G, E>), dim3(nb), dim3(256), 0, s, M, I, R, L, P, status, U, psum, pmax);
-    });
+    (void)L;
+    (void)status;
+    hipLaunchKernelGGL((k_apply<MT, UT>), dim3(nb), dim3(256), (size_t)R * (R + 2) * 8, s, M, I, R, P, U, psum, pmax);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }

@@ -35,7 +35,7 @@ static int g_variant = -1;
 static int g_graphs = 0;
 static int g_hot = 0;
 static int g_hot_block = 512;
-static int g_hot_lds = 49152;
+static int g_hot_lds = 32768;
 int graphs_enabled() { return g_graphs; }
 int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
@@ -155,6 +155,42 @@ static int check_offsets(const alto_tensor* t, int R)
     return ALTO_OK;
 }
 
+static size_t elem_bytes(int precision) { return precision == ALTO_PREC_F64 ? 8 : precision == ALTO_PREC_F32 ? 4 : 2; }
+
+// Stream layout for MTTKRPs of rank R in `precision`: per-mode streams,
+// rank-ordered factor buffers and, with hot rows on, each stream grouped by
+// hot pattern for the LDS budget.  A no-op once in place (so it may run
+// inside a graph capture after a first call outside it).
+int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s)
+{
+    ALTO_TRY(check_offsets(t, R));
+    ALTO_TRY(prepare_mode_streams(t, precision == ALTO_PREC_F64, precision != ALTO_PREC_F64, s));
+    const size_t eb = elem_bytes(precision);
+    ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, eb, s));
+    const int no = t->nmode - 1;
+    const int v = g_variant >= 0 ? g_variant : precision == ALTO_PREC_BF16 ? 1 : 2;
+    if (!g_hot || v != 2 || no < 1 || no > 2 || ((uint64_t)R * eb) % 16 != 0) return ALTO_OK;
+    for (int m = 0; m < t->nmode; ++m) {
+        if (!t->ms[m].ranked) continue;
+        int want[ALTO_MAXN - 1] = {0};
+        bool any = false;
+        int o = 0;
+        for (int q = 0; q < t->nmode; ++q) {
+            if (q == m) continue;
+            uint64_t rows = (uint64_t)g_hot_lds / no / ((uint64_t)R * eb);
+            if (rows > t->dims[q]) rows = t->dims[q];
+            want[o] = (int)rows;
+            any = any || rows > 0;
+            ++o;
+        }
+        if (!any) continue;
+        bool same = t->ms[m].split != 0;
+        for (int q = 0; q < no && same; ++q) same = t->ms[m].hot_k[q] == want[q];
+        if (!same) ALTO_TRY(split_stream(t, m, want, s));
+    }
+    return ALTO_OK;
+}
+
 int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* out, int precision, int accumulate,
                   hipStream_t s, int seg)
 {
@@ -162,11 +198,9 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
         set_error("mttkrp: mode %d / rank %d out of range (rank 1..128)", mode, R);
         return ALTO_ERR_ARG;
     }
-    ALTO_TRY(check_offsets(t, R));
-    ALTO_TRY(prepare_mode_streams(t, precision == ALTO_PREC_F64, precision != ALTO_PREC_F64, s));
+    ALTO_TRY(mttkrp_prepare(t, R, precision, s));
     uint64_t nout = t->dims[mode] * (uint64_t)R;
     if (seg <= 0) seg = default_seg(R);
-    ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, precision == ALTO_PREC_F64 ? 8 : precision == ALTO_PREC_F32 ? 4 : 2, s));
     MttkrpTune k;
     k.variant = g_variant;
     k.nt = g_nt;

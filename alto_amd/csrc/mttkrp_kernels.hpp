@@ -307,8 +307,8 @@ template <int G, int L> __device__ __forceinline__ float bcast_val(float v)
 // E = 16 / sizeof(FT) consecutive columns per lane in one 16-byte load; the
 // host guarantees 16-byte aligned rows (R * sizeof(FT) % 16 == 0), so a lane
 // either loads a whole vector or lies beyond the rank
-template <typename FT, typename AT, int E>
-__device__ __forceinline__ void load16(const FT* __restrict__ row, int gl, int R, AT (&x)[E])
+template <typename FT, typename AT, int E, typename P>
+__device__ __forceinline__ void load16(P row, int gl, int R, AT (&x)[E])
 {
     // branch-free: lanes beyond the rank read the row's first vector and
     // drop it
@@ -325,11 +325,16 @@ __device__ __forceinline__ void load16(const FT* __restrict__ row, int gl, int R
 // factor row arrives with one 16-byte load per lane.  The next batch's
 // nonzeros are loaded while the current batch's rows are gathered.
 // NO = number of other modes (a template constant, so that the gathers of
-// all modes are issued back to back with no branch between them).
-template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO>
+// all modes are issued back to back with no branch between them).  HM = hot
+// mask of a hot-pattern group of a split stream: for other mode o with bit
+// o set every nonzero's rank is below hot_n[o] / R, and the row is read from
+// the LDS copy of those rows (staged once per workgroup).
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO, int HM>
 __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
 {
     constexpr int J = G < 4 ? G : 4;  // nonzeros in flight per group
+    extern __shared__ __align__(16) unsigned char hot_smem[];
+    if constexpr (HM != 0) stage_hot_rows(a, hot_smem);
     const int lane = threadIdx.x & 63;
     const int gl = lane & (G - 1);
     const uint64_t block = xcd_block(a.nblocks_real);
@@ -380,11 +385,13 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
             static_for<0, NO>([&](auto oi) {
                 constexpr int o = decltype(oi)::value;
                 const FT* __restrict__ U = a.Uo[o];
+                const lds_ptr<FT> H = (lds_ptr<FT>)(hot_smem + a.hot_base[o]);
                 static_for<0, J>([&](auto ji) {
                     constexpr int jj = decltype(ji)::value;
                     const uint32_t off = bcast32<G, j0 + jj>(offs[o]);
                     AT x[E];
-                    load16<FT, AT, E>(U + off, gl, R, x);
+                    if constexpr ((HM >> o) & 1) load16<FT, AT, E>(H + off, gl, R, x);
+                    else load16<FT, AT, E>(U + off, gl, R, x);
 #pragma unroll
                     for (int q = 0; q < E; ++q) prod[jj][q] *= x[q];
                 });
@@ -447,19 +454,49 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     int v = k.variant >= 0 ? k.variant : std::is_same<FT, bf16_t>::value ? 1 : 2;
     const bool hot = t->ms[mode].ranked != 0 && a.nother > 0;
     constexpr int E16 = 16 / sizeof(FT);
-    if (v == 2 && (hot || !rows_16b_aligned(a, R) || R > 64 * E16 || a.nother < 1)) v = 0;
+    const ModeStream& M = t->ms[mode];
+    const bool split = M.split && hot && a.nother <= 2;
+    if (v == 2 && ((hot && !split) || !rows_16b_aligned(a, R) || R > 64 * E16 || a.nother < 1)) v = 0;
     if (v == 2) {
         int G = 4;
         while (G * E16 < R) G <<= 1;
-        dim3 grid = stream_grid(t->nnz, seg, G, a);
-#define K16N_(GG, PWW)                                                                                          \
-    do {                                                                                                        \
-        switch (a.nother) {                                                                                     \
-        case 1: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 1>), grid, dim3(256), 0, s, a); break; \
-        case 2: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 2>), grid, dim3(256), 0, s, a); break; \
-        case 3: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 3>), grid, dim3(256), 0, s, a); break; \
-        default: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, 4>), grid, dim3(256), 0, s, a); break; \
-        }                                                                                                       \
+        // one launch per hot-pattern group (one launch over the whole stream
+        // when it is not split)
+        const int npat = split ? 1 << a.nother : 1;
+        for (int p = 0; p < npat; ++p) {
+            StreamArgs<FT, VT, OT> b = a;
+            uint32_t lds = 0;
+            if (split) {
+                const uint64_t s0 = M.pat_start[p], cnt = M.pat_start[p + 1] - s0;
+                if (cnt == 0) continue;
+                b.pk = a.pk + s0 * t->pack.pw;
+                b.val = a.val + s0;
+                b.nnz = cnt;
+                for (int o = 0; o < a.nother; ++o) {
+                    b.hot_n[o] = (p >> o) & 1 ? (uint32_t)(M.hot_k[o] * R) : 0u;
+                    b.hot_base[o] = lds;
+                    lds += (uint32_t)((b.hot_n[o] * sizeof(FT) + 15) / 16 * 16);
+                }
+            }
+            if (lds > 160u * 1024u) {
+                set_error("mttkrp: %u bytes of staged rows exceed the 160 KiB of LDS (rank %d)", lds, R);
+                return ALTO_ERR_UNSUPPORTED;
+            }
+            dim3 grid = stream_grid(b.nnz, seg, G, b);
+#define K16L_(GG, PWW, NOO, HMM) \
+    hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, E16, PWW, NOO, HMM>), grid, dim3(256), lds, s, b)
+#define K16N_(GG, PWW)                                     \
+    do {                                                   \
+        switch (b.nother * 16 + p) {                       \
+        case 16: K16L_(GG, PWW, 1, 0); break;              \
+        case 17: K16L_(GG, PWW, 1, 1); break;              \
+        case 32: K16L_(GG, PWW, 2, 0); break;              \
+        case 33: K16L_(GG, PWW, 2, 1); break;              \
+        case 34: K16L_(GG, PWW, 2, 2); break;              \
+        case 35: K16L_(GG, PWW, 2, 3); break;              \
+        case 48: K16L_(GG, PWW, 3, 0); break;              \
+        default: K16L_(GG, PWW, 4, 0); break;              \
+        }                                                  \
     } while (0)
 #define K16_(GG)                                 \
     do {                                         \
@@ -468,14 +505,16 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
             else K16N_(GG, 2);                   \
         }                                        \
     } while (0)
-        K16_(4);
-        K16_(8);
-        K16_(16);
-        K16_(32);
-        K16_(64);
+            K16_(4);
+            K16_(8);
+            K16_(16);
+            K16_(32);
+            K16_(64);
 #undef K16_
 #undef K16N_
-        HIP_TRY(hipGetLastError());
+#undef K16L_
+            HIP_TRY(hipGetLastError());
+        }
         return ALTO_OK;
     }
     const int E = v == 1 ? 4 : 2;

@@ -119,11 +119,16 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * ALTO_TUNE_GRAPHS: 1 = CP-ALS sessions capture one iteration as a HIP graph
  * and replay it (non-verbose iterate calls; phase timers are then not kept).
  * ALTO_TUNE_HOT_ROWS: 1 = the streams address the other modes' factor rows
- * by frequency rank, and each MTTKRP workgroup stages the most frequent rows
- * of each input factor in LDS (strided lane mapping only); 0 (default) = rows
- * from global memory.  Streams are re-ordered on their next use.
- * ALTO_TUNE_HOT_BLOCK: threads per workgroup of that kernel (256/512/1024).
- * ALTO_TUNE_HOT_LDS: LDS bytes per workgroup for the staged rows. */
+ * by frequency rank and MTTKRP workgroups stage the most frequent rows of
+ * each input factor in LDS: with the 16-byte mapping on tensors of up to 3
+ * modes each stream is grouped by hot pattern (which of its rows are in
+ * LDS), one launch per group; with the other mappings each row read tests
+ * its rank.  0 (default) = rows from global memory.  Streams are re-ordered
+ * on their next use.
+ * ALTO_TUNE_HOT_BLOCK: threads per workgroup of the per-row-test kernel
+ * (256/512/1024).
+ * ALTO_TUNE_HOT_LDS: LDS bytes per workgroup for the staged rows (default
+ * 32768). */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
        ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9 };
 int alto_tune_set(int param, int value);

@@ -152,20 +152,20 @@ def tune_reset():
                          ids=lambda h: f"hot{h[0]}_bt{h[1]}_lds{h[2]}")
 @pytest.mark.parametrize("variant", [-1, 0, 1, 2])
 @pytest.mark.parametrize("chunk_log2", [0, 10])
-def test_mttkrp_tuning_knobs_keep_results(tune_reset, hot, variant, chunk_log2):
+@pytest.mark.parametrize("dims", [[300, 40, 700, 25], [300, 40, 700], [500, 3000]], ids=lambda d: f"{len(d)}m")
+def test_mttkrp_tuning_knobs_keep_results(tune_reset, hot, variant, chunk_log2, dims):
     """Rank-addressed streams with hot rows staged in LDS (on/off, workgroup
-    size, LDS budget including none), the lane mapping and the chunk size
-    change only the summation order: every combination matches the oracle in
-    FP64, FP32 and BF16, with the tolerances of the tests above.  Rank 7 and
-    100 rows are not 16-byte aligned in every precision (the 16-byte mapping
-    then falls back)."""
+    size, LDS budget including none; streams grouped by hot pattern for 2- and
+    3-mode tensors), the lane mapping and the chunk size change only the
+    summation order: every combination matches the oracle in FP64, FP32 and
+    BF16, with the tolerances of the tests above.  Rank 7 and 100 rows are not
+    16-byte aligned in every precision (the 16-byte mapping then falls back)."""
     A = tune_reset
     A.check(A.lib().alto_tune_set(7, hot[0]), "tune_set")
     A.check(A.lib().alto_tune_set(8, hot[1]), "tune_set")
     A.check(A.lib().alto_tune_set(9, hot[2]), "tune_set")
     A.check(A.lib().alto_tune_set(4, variant), "tune_set")
     A.check(A.lib().alto_tune_set(2, chunk_log2), "tune_set")
-    dims = [300, 40, 700, 25]
     cidx, vals = zipf_tensor(dims, 30000, 17)
     g = A.AltoTensor(dims, cidx, vals, nprtn=8)
     o = oracle.OracleAlto(dims, cidx, vals, W=1, nprtn=8)
