@@ -218,8 +218,7 @@ inline uint32_t plan_hot_rows(const alto_tensor* t, int mode, StreamArgs<FT, VT,
 
 // lanes per nonzero: 2 strided columns per lane, 4..64 lanes
 inline int group_lanes(int R) { return R <= 8 ? 4 : R <= 16 ? 8 : R <= 32 ? 16 : R <= 64 ? 32 : 64; }
-int tuned_seg(int R);
-inline int default_seg(int R) { return tuned_seg(R); }
+int tuned_seg(uint64_t nnz, int G);
 
 template <typename A>
 inline dim3 stream_grid(uint64_t nnz, int seg, int G, A& a, int block_threads = 256)

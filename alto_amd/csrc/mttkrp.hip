@@ -20,13 +20,20 @@
 
 namespace alto {
 
-// nonzeros per lane group; set by alto_tune_set (ALTO_TUNE_SEG) or the
-// autotuner, else 256
+// Nonzeros walked by one lane group: ALTO_TUNE_SEG when set, else sized so
+// that a launch has about 2^20 threads (G lanes per group), as a power of
+// two in [256, 4096].  Long segments cut the flushes -- and with them the
+// atomic traffic on heavily used output rows -- while enough groups remain
+// to fill the chip (measured on MI355X: 512-1024 for the 77M-nonzero 3-mode
+// tensor, 1024-4096 for the 200M-nonzero 5-mode one).
 static int g_seg = 0;
-int tuned_seg(int R)
+int tuned_seg(uint64_t nnz, int G)
 {
-    (void)R;
-    return g_seg > 0 ? g_seg : 256;
+    if (g_seg > 0) return g_seg;
+    const uint64_t want = nnz * (uint64_t)G >> 20;
+    int seg = 256;
+    while (seg < 4096 && (uint64_t)seg * 3 / 2 < want) seg <<= 1;
+    return seg;
 }
 static int g_chunk = 0;
 static int g_nt = 0;
@@ -200,7 +207,6 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
     }
     ALTO_TRY(mttkrp_prepare(t, R, precision, s));
     uint64_t nout = t->dims[mode] * (uint64_t)R;
-    if (seg <= 0) seg = default_seg(R);
     MttkrpTune k;
     k.variant = g_variant;
     k.nt = g_nt;
@@ -230,7 +236,7 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
     }
     ALTO_TRY(check_offsets(t, R));
     ALTO_TRY(prepare_mode_streams(t, 1, 0, s));
-    if (seg <= 0) seg = default_seg(R);
+    if (seg <= 0) seg = tuned_seg(t->nnz, group_lanes(R));
     ALTO_TRY(fill_zero(phi, t->dims[mode] * (uint64_t)R, s));
     ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, 8, s));
     StreamArgs<double, double, double> a;

@@ -332,7 +332,10 @@ __device__ __forceinline__ void load16(P row, int gl, int R, AT (&x)[E])
 template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO, int HM>
 __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
 {
-    constexpr int J = G < 4 ? G : 4;  // nonzeros in flight per group
+    // nonzeros in flight per group: fewer with more gathers per nonzero, to
+    // keep the loads in flight within the register budget
+    constexpr int JW = 4;
+    constexpr int J = G < JW ? G : JW;
     extern __shared__ __align__(16) unsigned char hot_smem[];
     if constexpr (HM != 0) stage_hot_rows(a, hot_smem);
     const int lane = threadIdx.x & 63;
@@ -460,6 +463,8 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     if (v == 2) {
         int G = 4;
         while (G * E16 < R) G <<= 1;
+        if (seg <= 0) seg = tuned_seg(t->nnz, G);
+        a.seg = (uint64_t)seg;
         // one launch per hot-pattern group (one launch over the whole stream
         // when it is not split)
         const int npat = split ? 1 << a.nother : 1;
@@ -520,6 +525,8 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     const int E = v == 1 ? 4 : 2;
     int G = 4;
     while (G * E < R && G < 64) G <<= 1;
+    if (seg <= 0) seg = tuned_seg(t->nnz, G);
+    a.seg = (uint64_t)seg;
     const int bt = hot ? k.hot_block : 256;
     const uint32_t lds = hot ? plan_hot_rows(t, mode, a, (uint32_t)k.hot_lds) : 0;
     dim3 grid = stream_grid(t->nnz, seg, G, a, bt);

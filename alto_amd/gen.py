@@ -60,11 +60,13 @@ def sparse_coo(dims, nnz, seed, dist="zipf", s=1.1, values="lognormal", device="
 
     uniq = None
     rounds = 0
+    yield_rate = 1.0  # new unique coordinates per draw in the last round
     while True:
         have = 0 if uniq is None else uniq.shape[0]
-        if have >= nnz or rounds > 60:
+        if have >= nnz or rounds > 200:
             break
-        m = int((nnz - have) * 1.3) + 1024
+        # skewed indices collide often: size the round by the observed yield
+        m = int(min((nnz - have) * 1.3 / max(yield_rate, 0.02), 2 * nnz)) + 1024
         cols = []
         for n, d in enumerate(dims):
             if dist == "zipf":
@@ -78,6 +80,7 @@ def sparse_coo(dims, nnz, seed, dist="zipf", s=1.1, values="lognormal", device="
         allk = k if uniq is None else torch.cat([uniq, k], 0)
         uniq = torch.unique(allk, dim=0) if allk.shape[1] > 1 else torch.unique(allk.squeeze(1)).unsqueeze(1)
         del allk, k, cols
+        yield_rate = (uniq.shape[0] - have) / m
         rounds += 1
     # seeded random subset of exactly nnz coordinates, in a shuffled order
     sel = torch.randperm(uniq.shape[0], generator=gen, device=device)[:nnz]

@@ -35,6 +35,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(nnz):
+    """HBM bytes per MTTKRP launch (read + write) from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this kernel and workload; None when it
+    does not describe this workload."""
+    path = os.path.join(ROOT, "profiles", "k_mttkrp16_nell2shape_zipf_r32_fp64.traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    if d.get("dims") != DIMS or d.get("nnz") != nnz or d.get("rank") != RANK:
+        return None
+    return d["hbm_read_bytes_per_launch"] + d["hbm_write_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,8 +185,8 @@ def main():
         "config": {"workload": "cp-als nell2-shape zipf", "dims": DIMS, "nnz": nnz, "rank": RANK,
                    "alto_key_bits": 64, "modes": 3, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_mttkrp (all-mode, 3 launches)", "ms_per_launch": mode_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nnz),
+                     "kernel": "k_mttkrp16 (all-mode, 3 launches)", "ms_per_launch": mode_ms,
                      "algorithmic_bytes_per_launch": mode_bytes},
         "mttkrp_allmode_ms": allmode_ms,
         "alto_create_s": create_s,

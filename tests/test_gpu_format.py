@@ -110,7 +110,7 @@ def test_mttkrp_fp32_bf16_error(R):
 
 
 @pytest.mark.parametrize("case", CASES[:6], ids=lambda c: f"{len(c['dims'])}m_{c['nnz']}")
-@pytest.mark.parametrize("R", [10, 16, 32])
+@pytest.mark.parametrize("R", [7, 10, 16, 32, 100])
 def test_phi_fp64(case, R):
     cidx, vals, g, o = _pair(case)
     rng = np.random.default_rng(R + 1)
