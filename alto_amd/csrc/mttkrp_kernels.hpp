@@ -271,8 +271,8 @@ __device__ __forceinline__ void static_for(F&& f)
 }
 
 // value of lane L (a constant) of each G-lane group, through VALU paths:
-// DPP quad_perm (G = 4), DPP row_newbcast (G = 8, 16: lane L of each 16-lane
-// row), readlane (G = 32, 64)
+// DPP quad_perm (G = 2, 4), DPP row_newbcast (G = 8, 16: lane L of each
+// 16-lane row), readlane (G = 32, 64); G = 1 is the lane itself
 template <int G, int L>
 __device__ __forceinline__ uint32_t bcast32(uint32_t v)
 {
@@ -288,9 +288,14 @@ __device__ __forceinline__ uint32_t bcast32(uint32_t v)
         const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + L, 0xf, 0xf, false);
         const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x150 + 8 + L, 0xf, 0xf, false);
         return (threadIdx.x & 8) ? hi : lo;
-    } else {
-        static_assert(G == 4, "group of 4..64 lanes");
+    } else if constexpr (G == 4) {
         return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, L | (L << 2) | (L << 4) | (L << 6), 0xf, 0xf, false);
+    } else if constexpr (G == 2) {  // quad_perm [L, L, 2+L, 2+L]
+        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, L | (L << 2) | ((L + 2) << 4) | ((L + 2) << 6), 0xf, 0xf,
+                                                     false);
+    } else {
+        static_assert(G == 1, "group of 1..64 lanes");
+        return v;
     }
 }
 template <int G, int L> __device__ __forceinline__ double bcast_val(double v)
@@ -332,10 +337,12 @@ __device__ __forceinline__ void load16(P row, int gl, int R, AT (&x)[E])
 template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO, int HM>
 __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
 {
-    // nonzeros in flight per group: fewer with more gathers per nonzero, to
-    // keep the loads in flight within the register budget
-    constexpr int JW = 4;
-    constexpr int J = G < JW ? G : JW;
+    // a batch is B = max(G, 4) nonzeros: each lane decodes NS = B / G of them
+    // (slot k of lane l holds nonzero p + k*G + l); J = 4 of them are in
+    // flight per group at a time
+    constexpr int NS = G < 4 ? 4 / G : 1;
+    constexpr int B = G * NS;
+    constexpr int J = 4;
     extern __shared__ __align__(16) unsigned char hot_smem[];
     if constexpr (HM != 0) stage_hot_rows(a, hot_smem);
     const int lane = threadIdx.x & 63;
@@ -353,35 +360,45 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
     for (int q = 0; q < E; ++q) acc[q] = 0;
     uint32_t cur = 0xffffffffu;
 
-    // lane gl's nonzero of the next batch (past the segment: row 0 of every
-    // mode, value 0 -- valid memory, and never accumulated)
-    uint64_t kn[PW];
-    VT vn = 0;
+    // this lane's nonzeros of the next batch (past the segment: row 0 of
+    // every mode, value 0 -- valid memory, and never accumulated)
+    uint64_t kn[NS][PW];
+    VT vn[NS];
 #pragma unroll
-    for (int w = 0; w < PW; ++w) kn[w] = 0;
-    if (s + gl < e) {
+    for (int k = 0; k < NS; ++k) {
+        const uint64_t i = s + k * G + gl;
+        vn[k] = 0;
 #pragma unroll
-        for (int w = 0; w < PW; ++w) kn[w] = a.pk[(s + gl) * PW + w];
-        vn = a.val[s + gl];
+        for (int w = 0; w < PW; ++w) kn[k][w] = 0;
+        if (i < e) {
+#pragma unroll
+            for (int w = 0; w < PW; ++w) kn[k][w] = a.pk[i * PW + w];
+            vn[k] = a.val[i];
+        }
     }
 
-    for (uint64_t p = s; p < e; p += G) {
-        uint32_t offs[NO > 0 ? NO : 1];
-        uint32_t rown;
-        const VT myv = vn;
+    for (uint64_t p = s; p < e; p += B) {
+        uint32_t offs[NS][NO > 0 ? NO : 1];
+        uint32_t rown[NS];
+        VT myv[NS];
 #pragma unroll
-        for (int o = 0; o < NO; ++o) offs[o] = (uint32_t)(field<PW>(kn, a.offo[o], a.bitso[o]) * (uint64_t)R);
-        rown = (uint32_t)field<PW>(kn, a.offn, a.bitsn);
-        const int cnt = (int)(e - p < (uint64_t)G ? e - p : (uint64_t)G);
-        static_for<0, G / J>([&](auto bi) {
+        for (int k = 0; k < NS; ++k) {
+            myv[k] = vn[k];
+#pragma unroll
+            for (int o = 0; o < NO; ++o) offs[k][o] = (uint32_t)(field<PW>(kn[k], a.offo[o], a.bitso[o]) * (uint64_t)R);
+            rown[k] = (uint32_t)field<PW>(kn[k], a.offn, a.bitsn);
+        }
+        const int cnt = (int)(e - p < (uint64_t)B ? e - p : (uint64_t)B);
+        static_for<0, B / J>([&](auto bi) {
             constexpr int j0 = decltype(bi)::value * J;
             if (j0 >= cnt) return;
             AT prod[J][E];
             uint32_t rj[J];
             static_for<0, J>([&](auto ji) {
                 constexpr int jj = decltype(ji)::value;
-                rj[jj] = bcast32<G, j0 + jj>(rown);
-                const AT v = (AT)bcast_val<G, j0 + jj>(myv);
+                constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
+                rj[jj] = bcast32<G, src>(rown[slot]);
+                const AT v = (AT)bcast_val<G, src>(myv[slot]);
 #pragma unroll
                 for (int q = 0; q < E; ++q) prod[jj][q] = v;
             });
@@ -391,7 +408,8 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
                 const lds_ptr<FT> H = (lds_ptr<FT>)(hot_smem + a.hot_base[o]);
                 static_for<0, J>([&](auto ji) {
                     constexpr int jj = decltype(ji)::value;
-                    const uint32_t off = bcast32<G, j0 + jj>(offs[o]);
+                    constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
+                    const uint32_t off = bcast32<G, src>(offs[slot][o]);
                     AT x[E];
                     if constexpr ((HM >> o) & 1) load16<FT, AT, E>(H + off, gl, R, x);
                     else load16<FT, AT, E>(U + off, gl, R, x);
@@ -401,14 +419,17 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
             });
             if constexpr (j0 == 0) {
                 // the next batch's nonzeros, issued behind this block's gathers
-                const uint64_t pn = p + G + gl;
 #pragma unroll
-                for (int w = 0; w < PW; ++w) kn[w] = 0;
-                vn = 0;
-                if (pn < e) {
+                for (int k = 0; k < NS; ++k) {
+                    const uint64_t i = p + B + k * G + gl;
+                    vn[k] = 0;
 #pragma unroll
-                    for (int w = 0; w < PW; ++w) kn[w] = a.pk[pn * PW + w];
-                    vn = a.val[pn];
+                    for (int w = 0; w < PW; ++w) kn[k][w] = 0;
+                    if (i < e) {
+#pragma unroll
+                        for (int w = 0; w < PW; ++w) kn[k][w] = a.pk[i * PW + w];
+                        vn[k] = a.val[i];
+                    }
                 }
             }
 #pragma unroll
@@ -461,7 +482,7 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     const bool split = M.split && hot && a.nother <= 2;
     if (v == 2 && ((hot && !split) || !rows_16b_aligned(a, R) || R > 64 * E16 || a.nother < 1)) v = 0;
     if (v == 2) {
-        int G = 4;
+        int G = 1;
         while (G * E16 < R) G <<= 1;
         if (seg <= 0) seg = tuned_seg(t->nnz, G);
         a.seg = (uint64_t)seg;
@@ -510,6 +531,8 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
             else K16N_(GG, 2);                   \
         }                                        \
     } while (0)
+            K16_(1);
+            K16_(2);
             K16_(4);
             K16_(8);
             K16_(16);
