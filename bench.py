@@ -328,13 +328,15 @@ def serve_leg(n_jobs):
     for name in ("fp32_batched", "bf16_batched"):
         out[name + "_speedup_vs_fp64"] = out[name]["jobs_per_s"] / out["fp64_per_job"]["jobs_per_s"]
     # saturation capacity: every job available at t = 0
-    for name, prec, slots in (("fp64_per_job_saturated", PREC_F64, 1), ("bf16_batched_saturated", PREC_BF16, 8)):
+    for name, prec, slots in (("fp64_per_job_saturated", PREC_F64, 1), ("fp32_batched_saturated", PREC_F32, 8),
+                              ("bf16_batched_saturated", PREC_BF16, 8)):
         r = run_stream(jobs, prec, slots, time_scale=0.0)
         r.pop("fits")
         out[name] = r
         log(f"serving {name}: {r['jobs_per_s']:.1f} jobs/s")
-    out["bf16_saturated_speedup_vs_fp64"] = (out["bf16_batched_saturated"]["jobs_per_s"] /
-                                             out["fp64_per_job_saturated"]["jobs_per_s"])
+    for name in ("fp32", "bf16"):
+        out[name + "_saturated_speedup_vs_fp64"] = (out[name + "_batched_saturated"]["jobs_per_s"] /
+                                                    out["fp64_per_job_saturated"]["jobs_per_s"])
     del jobs
     torch.cuda.empty_cache()
     return out
