@@ -218,7 +218,7 @@ inline uint32_t plan_hot_rows(const alto_tensor* t, int mode, StreamArgs<FT, VT,
 
 // lanes per nonzero: 2 strided columns per lane, 4..64 lanes
 inline int group_lanes(int R) { return R <= 8 ? 4 : R <= 16 ? 8 : R <= 32 ? 16 : R <= 64 ? 32 : 64; }
-int tuned_seg(uint64_t nnz, int G);
+int tuned_seg(uint64_t nnz, int G, int acc_bytes);
 
 template <typename A>
 inline dim3 stream_grid(uint64_t nnz, int seg, int G, A& a, int block_threads = 256)

@@ -20,18 +20,19 @@
 
 namespace alto {
 
-// Nonzeros walked by one lane group: ALTO_TUNE_SEG when set, else sized so
-// that a launch has about 2^20 threads (G lanes per group), as a power of
-// two in [256, 4096].  Long segments cut the flushes -- and with them the
-// atomic traffic on heavily used output rows -- while enough groups remain
-// to fill the chip (measured on MI355X: 512-1024 for the 77M-nonzero 3-mode
-// tensor, 1024-4096 for the 200M-nonzero 5-mode one).
+// Nonzeros walked by one lane group: ALTO_TUNE_SEG when set, else a power
+// of two in [512, 4096] (from 256 below 4M nonzeros) sized so that a launch
+// has about 2^21 threads (G lanes per group), doubled for FP32 accumulation.  Long segments cut the
+// flushes -- and with them the atomic traffic on heavily used output rows
+// -- while enough groups remain to fill the chip (measured on MI355X: 512
+// for the 77M-nonzero 3-mode tensor in FP64, 1024 in FP32; 1024-4096 for the
+// 200M-nonzero 5-mode one; 512 for 14M-nonzero rank-8 tensors).
 static int g_seg = 0;
-int tuned_seg(uint64_t nnz, int G)
+int tuned_seg(uint64_t nnz, int G, int acc_bytes)
 {
     if (g_seg > 0) return g_seg;
-    const uint64_t want = nnz * (uint64_t)G >> 20;
-    int seg = 256;
+    const uint64_t want = (nnz * (uint64_t)G >> 21) * (acc_bytes < 8 ? 2 : 1);
+    int seg = nnz < (4ull << 20) ? 256 : 512;  // small tensors keep more groups
     while (seg < 4096 && (uint64_t)seg * 3 / 2 < want) seg <<= 1;
     return seg;
 }
@@ -236,7 +237,7 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
     }
     ALTO_TRY(check_offsets(t, R));
     ALTO_TRY(prepare_mode_streams(t, 1, 0, s));
-    if (seg <= 0) seg = tuned_seg(t->nnz, group_lanes(R));
+    if (seg <= 0) seg = tuned_seg(t->nnz, group_lanes(R), 8);
     ALTO_TRY(fill_zero(phi, t->dims[mode] * (uint64_t)R, s));
     ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, 8, s));
     StreamArgs<double, double, double> a;

@@ -484,7 +484,7 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     if (v == 2) {
         int G = 1;
         while (G * E16 < R) G <<= 1;
-        if (seg <= 0) seg = tuned_seg(t->nnz, G);
+        if (seg <= 0) seg = tuned_seg(t->nnz, G, (int)sizeof(AT));
         a.seg = (uint64_t)seg;
         // one launch per hot-pattern group (one launch over the whole stream
         // when it is not split)
@@ -548,7 +548,7 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     const int E = v == 1 ? 4 : 2;
     int G = 4;
     while (G * E < R && G < 64) G <<= 1;
-    if (seg <= 0) seg = tuned_seg(t->nnz, G);
+    if (seg <= 0) seg = tuned_seg(t->nnz, G, (int)sizeof(AT));
     a.seg = (uint64_t)seg;
     const int bt = hot ? k.hot_block : 256;
     const uint32_t lds = hot ? plan_hot_rows(t, mode, a, (uint32_t)k.hot_lds) : 0;
