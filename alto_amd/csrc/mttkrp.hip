@@ -176,8 +176,9 @@ int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s)
     const size_t eb = elem_bytes(precision);
     ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, eb, s));
     const int no = t->nmode - 1;
-    const int v = g_variant >= 0 ? g_variant : precision == ALTO_PREC_BF16 ? 1 : 2;
-    if (!g_hot || v != 2 || no < 1 || no > 2 || ((uint64_t)R * eb) % 16 != 0) return ALTO_OK;
+    const int v = g_variant >= 0 ? g_variant : 2;
+    const uint64_t lane_bytes = precision == ALTO_PREC_BF16 ? 8 : 16;
+    if (!g_hot || v != 2 || no < 1 || no > 2 || ((uint64_t)R * eb) % lane_bytes != 0) return ALTO_OK;
     for (int m = 0; m < t->nmode; ++m) {
         if (!t->ms[m].ranked) continue;
         int want[ALTO_MAXN - 1] = {0};

@@ -450,22 +450,22 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
     if (cur != 0xffffffffu) flush_row<AT, OT, G, E, true>(a.out, cur, R, gl, acc);
 }
 
-// the 16-byte-lane kernel needs 16-byte aligned factor rows
+// the vector-lane kernel needs factor rows aligned to the lane's load width
 template <typename FT, typename VT, typename OT>
-inline bool rows_16b_aligned(const StreamArgs<FT, VT, OT>& a, int R)
+inline bool rows_aligned(const StreamArgs<FT, VT, OT>& a, int R, int bytes)
 {
-    if (((uint64_t)R * sizeof(FT)) % 16 != 0) return false;
+    if (((uint64_t)R * sizeof(FT)) % bytes != 0) return false;
     for (int o = 0; o < a.nother; ++o)
-        if ((uintptr_t)a.Uo[o] % 16 != 0) return false;
+        if ((uintptr_t)a.Uo[o] % bytes != 0) return false;
     return true;
 }
 
 // Lane mappings: 0 = 2 strided columns per lane (each gather instruction
 // reads G consecutive elements of a row); 1 = 4 contiguous columns per lane;
-// 2 = E = 16 / sizeof(FT) contiguous columns per lane in one 16-byte load,
-// broadcasts through DPP / readlane.  -1 picks per element type, from
-// measurements on MI355X at rank 32: FP64 and FP32 -> 2, BF16 -> 1.
-// Ranked streams (hot rows in LDS) use mapping 0 or 1.
+// 2 = contiguous columns per lane in one vector load (16 bytes; 8 for BF16),
+// broadcasts through DPP / readlane.  -1 = 2 (measured fastest on MI355X for
+// every element type).  Ranked streams without the hot-pattern split use
+// mapping 0 or 1.
 template <typename FT, typename VT, typename AT, typename OT>
 int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, const VT* val, OT* out, int seg,
                   const MttkrpTune& k, hipStream_t s)
@@ -475,12 +475,15 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     ALTO_TRY(set_factors(t, mode, U, a, s));
     a.val = val;
     a.out = out;
-    int v = k.variant >= 0 ? k.variant : std::is_same<FT, bf16_t>::value ? 1 : 2;
+    int v = k.variant >= 0 ? k.variant : 2;
     const bool hot = t->ms[mode].ranked != 0 && a.nother > 0;
-    constexpr int E16 = 16 / sizeof(FT);
+    // elements per lane of the vector-lane mapping: one 16-byte load (FP64,
+    // FP32), one 8-byte load for BF16 (four columns: eight would need ~110
+    // VGPRs and halve the occupancy)
+    constexpr int E16 = std::is_same<FT, bf16_t>::value ? 4 : 16 / sizeof(FT);
     const ModeStream& M = t->ms[mode];
     const bool split = M.split && hot && a.nother <= 2;
-    if (v == 2 && ((hot && !split) || !rows_16b_aligned(a, R) || R > 64 * E16 || a.nother < 1)) v = 0;
+    if (v == 2 && ((hot && !split) || !rows_aligned(a, R, E16 * (int)sizeof(FT)) || R > 64 * E16 || a.nother < 1)) v = 0;
     if (v == 2) {
         int G = 1;
         while (G * E16 < R) G <<= 1;

@@ -113,9 +113,9 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * (0 = default); streams are re-ordered on their next use.
  * ALTO_TUNE_NONTEMPORAL: 1 = stream the nonzeros with non-temporal loads.
  * ALTO_TUNE_MTTKRP_VARIANT: lane mapping of the MTTKRP kernel: 0 = 2 strided
- * columns per lane, 1 = 4 contiguous columns per lane, 2 = one 16-byte load
- * per lane and row (needs rank * element size % 16 == 0, else 0 is used);
- * -1 (default) = chosen per factor element type.
+ * columns per lane, 1 = 4 contiguous columns per lane, 2 = one vector load
+ * per lane and row, 16 bytes (8 for BF16) (needs rows aligned to that width,
+ * else 0 is used); -1 (default) = 2.
  * ALTO_TUNE_GRAPHS: 1 = CP-ALS sessions capture one iteration as a HIP graph
  * and replay it (non-verbose iterate calls; phase timers are then not kept).
  * ALTO_TUNE_HOT_ROWS: 1 = the streams address the other modes' factor rows
