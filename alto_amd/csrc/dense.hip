@@ -4,13 +4,13 @@
 //    factors, v_mfma_f32_16x16x4f32 for FP32 / BF16 factors; per-block
 //    partial Grams reduced in a fixed order (deterministic).
 //    Reference: update_gram = SYRK (src/kruskal_model.cpp:11-24).
-//  - V = Hadamard product of the other modes' Grams and its Cholesky factor
-//    in one workgroup (FP64); when Cholesky fails, the minimum-norm
-//    pseudo-inverse from a Jacobi eigen-decomposition.
+//  - V = Hadamard product of the other modes' Grams and V^-1 by Gauss-Jordan
+//    elimination in one workgroup (FP64; its pivots fail exactly when a
+//    Cholesky factorization would); then the minimum-norm pseudo-inverse
+//    from a Jacobi eigen-decomposition.
 //    Reference: pseudo_inverse (include/cpd.hpp:546-646).
 //  - Row solves U_n[i,:] = M[i,:] V^-1 as a GEMM with the explicit inverse
-//    V^-1 = L^-T L^-1 from the Cholesky factor (or the pseudo-inverse),
-//    MFMA FP64, fused with the per-block column norms.
+//    (or the pseudo-inverse), MFMA FP64, fused with the per-block column norms.
 //  - Column normalisation (2-norm / max-norm), the fit, ||X||^2.
 //    Reference: KruskalModelNorm (src/kruskal_model.cpp:224-341),
 //    cpd_fit_alto (include/cpd.hpp:189-271).
@@ -123,17 +123,19 @@ template int gram_device<float>(const float*, uint64_t, int, double*, double*, h
 template int gram_device<bf16_t>(const bf16_t*, uint64_t, int, double*, double*, hipStream_t, bool);
 
 // ------------------------------------------------------------------ V and its Cholesky factor
-// V = prod_{m != mode} G_m (elementwise; all ones for a 1-mode tensor).
-// Right-looking Cholesky of the lower triangle in LDS (R <= 128, one
-// workgroup of 256 threads; the trailing update runs on a 32 x 8 thread
-// grid: columns on x, rows on y).  status: 0 = Cholesky (L in Lout), 1 =
-// not positive definite (the pseudo-inverse path takes over).
-__global__ void __launch_bounds__(256) k_chol(const double* __restrict__ grams, int nmode, int mode, int R,
-                                              double* __restrict__ Lout, int* __restrict__ status)
+// V^-1 for symmetric positive definite V = prod_{m != mode} G_m by in-place
+// Gauss-Jordan elimination without pivoting in LDS (one workgroup, R <= 128):
+// the k-th pivot is the k-th Schur complement diagonal, i.e. the square of
+// the Cholesky diagonal, so a pivot <= 0 is exactly the Cholesky failure of
+// the reference's dpotrf test; then status = 1 and the pseudo-inverse path
+// (k_pinv) takes over.  status 0: W = V^-1.
+__global__ void __launch_bounds__(256) k_spd_inverse(const double* __restrict__ grams, int nmode, int mode, int R,
+                                                     double* __restrict__ W, int* __restrict__ status)
 {
-    extern __shared__ double A[];
+    extern __shared__ double A[];  // R x R, row stride R
+    __shared__ double s_col[128];
     __shared__ int ok;
-    const int tid = threadIdx.x, tx = tid & 31, ty = tid >> 5;
+    const int tid = threadIdx.x;
     for (int k = tid; k < R * R; k += blockDim.x) {
         double v = 1.0;
         for (int m = 0; m < nmode; ++m)
@@ -142,58 +144,35 @@ __global__ void __launch_bounds__(256) k_chol(const double* __restrict__ grams, 
     }
     if (tid == 0) ok = 1;
     __syncthreads();
-    for (int j = 0; j < R; ++j) {
-        if (tid == 0) {
-            const double d = A[j * R + j];
-            if (!(d > 0.0)) ok = 0;
-            else A[j * R + j] = sqrt(d);
+    // elimination layout: thread -> column jc, rows tid / R + rstep * n
+    const int rstep = (int)blockDim.x / R, jc = tid % R;
+    for (int k = 0; k < R; ++k) {
+        const double piv = A[k * R + k];
+        if (!(piv > 0.0)) {  // uniform: every thread read the same pivot
+            if (tid == 0) ok = 0;
+            break;
         }
+        // column k of the other rows (the elimination factors), then row k
+        // scaled by 1 / pivot with its diagonal slot holding 1 / pivot
+        for (int i = tid; i < R; i += blockDim.x) s_col[i] = i == k ? 0.0 : A[i * R + k];
         __syncthreads();
-        if (!ok) break;
-        const double djj = A[j * R + j];
-        for (int i = j + 1 + tid; i < R; i += blockDim.x) A[i * R + j] /= djj;
+        for (int j = tid; j < R; j += blockDim.x) A[k * R + j] = (j == k ? 1.0 : A[k * R + j]) / piv;
         __syncthreads();
-        for (int i = j + 1 + ty; i < R; i += 8) {
-            const double aij = A[i * R + j];
-            for (int k = j + 1 + tx; k <= i; k += 32) A[i * R + k] -= aij * A[k * R + j];
+        if (tid < rstep * R) {
+            const double akj = A[k * R + jc];
+            for (int i = tid / R; i < R; i += rstep) {
+                if (i == k) continue;
+                double* a = &A[i * R + jc];
+                *a = (jc == k ? 0.0 : *a) - s_col[i] * akj;
+            }
         }
         __syncthreads();
     }
+    __syncthreads();
     if (ok) {
-        for (int k = tid; k < R * R; k += blockDim.x) {
-            const int i = k / R, c = k % R;
-            Lout[k] = c <= i ? A[k] : 0.0;
-        }
+        for (int e = tid; e < R * R; e += blockDim.x) W[e] = A[e];
     }
     if (tid == 0) *status = ok ? 0 : 1;
-}
-
-// V^-1 = L^-T L^-1 from the Cholesky factor (status 0) into W; with status 1
-// W already holds the pseudo-inverse.  One workgroup: the columns of L^-1 by
-// forward substitution (a thread per column), then the product.
-__global__ void __launch_bounds__(256) k_chol_inverse(const double* __restrict__ L, int R, const int* __restrict__ status,
-                                                      double* __restrict__ W)
-{
-    if (*status != 0) return;
-    extern __shared__ double Li[];  // L^-1, row stride R + 1 (L itself is read from global memory)
-    const int LS = R + 1;
-    for (int k = threadIdx.x; k < R * R; k += blockDim.x) Li[(k / R) * LS + k % R] = 0.0;
-    __syncthreads();
-    for (int j = threadIdx.x; j < R; j += blockDim.x) {
-        Li[j * LS + j] = 1.0 / L[j * R + j];
-        for (int i = j + 1; i < R; ++i) {
-            double acc = 0;
-            for (int k = j; k < i; ++k) acc += L[i * R + k] * Li[k * LS + j];
-            Li[i * LS + j] = -acc / L[i * R + i];
-        }
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < R * R; e += blockDim.x) {
-        const int a = e / R, b = e % R;
-        double acc = 0;
-        for (int k = a > b ? a : b; k < R; ++k) acc += Li[k * LS + a] * Li[k * LS + b];
-        W[e] = acc;
-    }
 }
 
 // Minimum-norm pseudo-inverse V^+ = Q diag(1/e) Q^T over |e| > R*eps*max|e|
@@ -544,9 +523,9 @@ int chol_device(const double* grams, int nmode, int mode, int R, double* L, doub
         set_error("pseudo-inverse: rank above 128");
         return ALTO_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(k_chol, dim3(1), dim3(256), (size_t)R * R * 8, s, grams, nmode, mode, R, L, status);
+    (void)L;
+    hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(256), (size_t)R * R * 8, s, grams, nmode, mode, R, P, status);
     hipLaunchKernelGGL(k_pinv, dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, wsQ, P, status);
-    hipLaunchKernelGGL(k_chol_inverse, dim3(1), dim3(256), (size_t)R * (R + 1) * 8, s, L, R, status, P);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }
