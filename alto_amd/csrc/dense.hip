@@ -86,6 +86,73 @@ __global__ void __launch_bounds__(256) k_gram_partial(const T* __restrict__ U, u
     }
 }
 
+// Ranks up to 64 (T16 <= 4 column blocks of 16): each wave owns 4-row steps
+// of the block's rows, loads each 16-column block of a step once (lane l:
+// U[i0 + l/16][16 cb + l%16]) and multiplies every block pair ta <= tb into
+// its accumulators (the Gram is symmetric); the 4 waves' tiles are summed in
+// LDS and the block writes both triangles of its partial.
+template <typename T, bool F64, int T16>
+__global__ void __launch_bounds__(256) k_gram_rows(const T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
+                                                   double* __restrict__ part)
+{
+    constexpr int NP = T16 * (T16 + 1) / 2;
+    using acc_t = typename std::conditional<F64, d4_t, f4_t>::type;
+    extern __shared__ double sG[];  // R x R
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    acc_t acc[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) acc[q] = acc_t{0, 0, 0, 0};
+    const uint64_t base = (uint64_t)blockIdx.x * rows_per_block;
+    const uint64_t end = base + rows_per_block < I ? base + rows_per_block : I;
+    for (uint64_t i0 = base + 4 * wave; i0 < end; i0 += 16) {
+        const uint64_t i = i0 + kq;
+        double x[T16];
+#pragma unroll
+        for (int cb = 0; cb < T16; ++cb) {
+            const int c = cb * 16 + r16;
+            x[cb] = (i < end && c < R) ? as_f64(U[i * R + c]) : 0.0;
+        }
+        int q = 0;
+#pragma unroll
+        for (int ta = 0; ta < T16; ++ta)
+#pragma unroll
+            for (int tb = ta; tb < T16; ++tb, ++q) {
+                if constexpr (F64)
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[ta], x[tb], acc[q], 0, 0, 0);
+                else
+                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)x[ta], (float)x[tb], acc[q], 0, 0, 0);
+            }
+    }
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) sG[k] = 0.0;
+    __syncthreads();
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+            int q = 0;
+#pragma unroll
+            for (int ta = 0; ta < T16; ++ta)
+#pragma unroll
+                for (int tb = ta; tb < T16; ++tb, ++q) {
+                    const int b = tb * 16 + r16;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        // v_mfma_f64_16x16x4f64: lane l, register v holds row l/16 + 4v;
+                        // v_mfma_f32_16x16x4f32: row 4*(l/16) + v (both column l%16)
+                        const int a = ta * 16 + (F64 ? kq + 4 * v : 4 * kq + v);
+                        if (a < R && b < R) sG[a * R + b] += (double)acc[q][v];
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    double* P = part + (size_t)blockIdx.x * R * R;
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
+        const int a = k / R, b = k % R;
+        // the upper block triangle holds the sums; mirror the lower one
+        P[k] = (a / 16 <= b / 16) ? sG[k] : sG[b * R + a];
+    }
+}
+
 __global__ void k_sum_partials(const double* __restrict__ part, int nparts, int n, double* __restrict__ out)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -110,10 +177,26 @@ int gram_device(const T* U, uint64_t I, int R, double* G, double* part, hipStrea
         return ALTO_ERR_UNSUPPORTED;
     }
     int nb = gram_blocks(I);
-    if (f64)
+    if (R <= 64) {
+        const uint64_t rpb = ((I + nb - 1) / nb + 15) / 16 * 16;
+        const size_t lds = (size_t)R * R * 8;
+#define GR_(TT)                                                                                                      \
+    do {                                                                                                             \
+        if (f64) hipLaunchKernelGGL((k_gram_rows<T, true, TT>), dim3(nb), dim3(256), lds, s, U, I, R, rpb, part);  \
+        else hipLaunchKernelGGL((k_gram_rows<T, false, TT>), dim3(nb), dim3(256), lds, s, U, I, R, rpb, part);     \
+    } while (0)
+        switch ((R + 15) / 16) {
+        case 1: GR_(1); break;
+        case 2: GR_(2); break;
+        case 3: GR_(3); break;
+        default: GR_(4); break;
+        }
+#undef GR_
+    } else if (f64) {
         hipLaunchKernelGGL((k_gram_partial<T, true>), dim3(nb), dim3(256), 0, s, U, I, R, part);
-    else
+    } else {
         hipLaunchKernelGGL((k_gram_partial<T, false>), dim3(nb), dim3(256), 0, s, U, I, R, part);
+    }
     hipLaunchKernelGGL(k_sum_partials, dim3((R * R + 255) / 256), dim3(256), 0, s, part, nb, R * R, G);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
