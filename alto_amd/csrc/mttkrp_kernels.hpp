@@ -484,6 +484,40 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     const ModeStream& M = t->ms[mode];
     const bool split = M.split && hot && a.nother <= 2;
     if (v == 2 && ((hot && !split) || !rows_aligned(a, R, E16 * (int)sizeof(FT)) || R > 64 * E16 || a.nother < 1)) v = 0;
+    // FP32 / BF16 rows of rank <= 8: two columns per lane, so that a wave
+    // holds at most 16 groups (with 4 columns per lane it would hold 32, and
+    // each row change of any group costs the wave a round of atomics)
+    constexpr bool NARROW_OK = !std::is_same<FT, double>::value;
+    if (v == 2 && NARROW_OK && R <= 8 && !split && a.nother >= 1 && rows_aligned(a, R, 2 * (int)sizeof(FT))) {
+        int G = 1;
+        while (G * 2 < R) G <<= 1;
+        if (seg <= 0) seg = tuned_seg(t->nnz, G, (int)sizeof(AT));
+        a.seg = (uint64_t)seg;
+        dim3 grid = stream_grid(t->nnz, seg, G, a);
+#define KN_(GG, PWW)                                                                                            \
+    do {                                                                                                        \
+        switch (a.nother) {                                                                                     \
+        case 1: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, 2, PWW, 1, 0>), grid, dim3(256), 0, s, a); break; \
+        case 2: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, 2, PWW, 2, 0>), grid, dim3(256), 0, s, a); break; \
+        case 3: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, 2, PWW, 3, 0>), grid, dim3(256), 0, s, a); break; \
+        default: hipLaunchKernelGGL((k_mttkrp16<FT, VT, AT, OT, GG, 2, PWW, 4, 0>), grid, dim3(256), 0, s, a); break; \
+        }                                                                                                       \
+    } while (0)
+#define KNG_(GG)                                 \
+    do {                                         \
+        if (G == GG) {                           \
+            if (t->pack.pw == 1) KN_(GG, 1);     \
+            else KN_(GG, 2);                     \
+        }                                        \
+    } while (0)
+        KNG_(1);
+        KNG_(2);
+        KNG_(4);
+#undef KNG_
+#undef KN_
+        HIP_TRY(hipGetLastError());
+        return ALTO_OK;
+    }
     if (v == 2) {
         int G = 1;
         while (G * E16 < R) G <<= 1;
