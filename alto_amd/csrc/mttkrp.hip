@@ -44,6 +44,7 @@ static int g_graphs = 0;
 static int g_hot = 0;
 static int g_hot_block = 512;
 static int g_hot_lds = 32768;
+static int g_narrow_max = 16;
 int graphs_enabled() { return g_graphs; }
 int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
@@ -214,6 +215,7 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
     k.nt = g_nt;
     k.hot_block = g_hot_block;
     k.hot_lds = g_hot_lds;
+    k.narrow_max = g_narrow_max;
     if (precision == ALTO_PREC_F64) {
         if (!accumulate) ALTO_TRY(fill_zero((double*)out, nout, s));
         return launch_mttkrp_f64(t, mode, R, (const double* const*)U, t->ms[mode].val64.as<double>(), (double*)out, seg,
@@ -262,6 +264,11 @@ int alto_tune_set(int param, int value)
 {
     if (param == ALTO_TUNE_SEG) return set_seg(value);
     if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
+    if (param == ALTO_TUNE_NARROW_RANK) {
+        if (value < 0 || value > 16) return ALTO_ERR_ARG;
+        g_narrow_max = value;
+        return ALTO_OK;
+    }
     if (param == ALTO_TUNE_HOT_ROWS) {
         g_hot = value ? 1 : 0;
         return ALTO_OK;

