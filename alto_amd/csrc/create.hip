@@ -644,9 +644,11 @@ __global__ void __launch_bounds__(256) k_cr_owner(const uint64_t* __restrict__ k
         for (int w = 0; w < W; ++w) k[w] = keys[i * W + w];
         lut_apply<W, PW>(lut, g_lut.nb, k, pk);
         uint64_t r = field<PW>(pk, L.off[mode], L.bits[mode]);
-        int p = (int)(i / per);
-        atomicMin(&lo[r], p);
-        atomicMax(&hi[r], p);
+        const int p = (int)(i / per);
+        // most updates change nothing (many nonzeros per row and partition):
+        // read before issuing the atomic
+        if (lo[r] > p) atomicMin(&lo[r], p);
+        if (hi[r] < p) atomicMax(&hi[r], p);
     }
 }
 
@@ -1186,7 +1188,8 @@ static int create_impl(int nmode, const uint64_t* dims, uint64_t nnz, const uint
     }
     for (int m = 0; m < nmode; ++m) {
         uint64_t cm0 = t->cr_masks[m * t->W], cm1 = t->W == 2 ? t->cr_masks[m * t->W + 1] : 0;
-        if (!(cm0 | cm1)) continue;
+        // with one partition no row is shared, so no key gets the bit
+        if (!(cm0 | cm1) || nprtn == 1) continue;
         DevBuf lo, hi;
         ALTO_TRY(lo.alloc(dims[m] * 4));
         ALTO_TRY(hi.alloc(dims[m] * 4));

@@ -19,6 +19,7 @@ CASES = [
     dict(dims=[100, 90, 80, 70, 60], nnz=4000, P=3, W=1, seed=6),
     dict(dims=[1, 7, 1], nnz=7, P=2, W=1, seed=7),  # singleton modes (0 index bits)
     dict(dims=[3, 4, 5], nnz=1, P=1, W=1, seed=8),  # a single nonzero
+    dict(dims=[5000, 4000, 3000], nnz=4000, P=1, W=1, seed=9),  # low reuse, one partition: CR masks, no CR bits
 ]
 
 
@@ -169,7 +170,7 @@ def test_mttkrp_tuning_knobs_keep_results(tune_reset, hot, variant, chunk_log2, 
     cidx, vals = zipf_tensor(dims, 30000, 17)
     g = A.AltoTensor(dims, cidx, vals, nprtn=8)
     o = oracle.OracleAlto(dims, cidx, vals, W=1, nprtn=8)
-    for R in (7, 8, 32, 64, 100):
+    for R in (4, 7, 8, 16, 32, 64, 100):
         rng = np.random.default_rng(R)
         U = [rng.random((d, R)) for d in dims]
         for mode in range(len(dims)):
