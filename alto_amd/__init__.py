@@ -38,6 +38,11 @@ class TensorInfo(C.Structure):
                 ("alto_cr_mask", C.c_uint64 * 2), ("device_bytes", C.c_uint64)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("mode", C.c_int), ("variant", C.c_int), ("seg", C.c_int), ("ms", C.c_double),
+                ("default_ms", C.c_double)]
+
+
 class CpdTimers(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("total", "mttkrp", "pseudoinv", "memcopy", "normalize", "update", "fit")]
 
@@ -81,6 +86,7 @@ def lib():
         "alto_cpd_session_destroy": (i32, [vp]),
         "alto_pseudo_inverse": (i32, [vp, i32, i32, u64, vp, u64, C.POINTER(i32), vp]),
         "alto_tune_set": (i32, [i32, i32]),
+        "alto_mttkrp_autotune": (i32, [vp, u64, i32, vp, vp, i32, vp, vp]),
         "alto_kruskal_random_init": (i32, [i32, vp, u64, C.c_uint, vp, vp]),
         "alto_device_malloc": (i32, [C.POINTER(vp), u64]),
         "alto_device_free": (i32, [vp]),
@@ -238,6 +244,29 @@ class AltoTensor:
         ptrs = _ptr_array([f.ptr for f in factors])
         check(lib().alto_mttkrp(self.h, mode, factors[0].shape[1], ptrs, out.ptr, precision, int(accumulate), stream),
               "alto_mttkrp")
+
+    def mttkrp_autotune(self, rank, precision=PREC_F64, reps=3, factor_ptrs=None, out_ptr=None, stream=None):
+        """alto_mttkrp_autotune: picks the MTTKRP launch shape per mode for this
+        (rank, precision) and keeps it on the tensor.  Without device factors
+        given, random ones of that precision are used.  Returns one dict per mode."""
+        keep = []
+        if factor_ptrs is None:
+            rng = np.random.default_rng(0)
+            for d in self.dims:
+                u = rng.random((int(d), rank))
+                a = (DeviceArray.from_numpy(u) if precision == PREC_F64 else
+                     DeviceArray.from_numpy(u.astype(np.float32)) if precision == PREC_F32 else
+                     DeviceArray.from_numpy(to_bf16_bits(u)))
+                keep.append(a)
+            factor_ptrs = [a.ptr for a in keep]
+        if out_ptr is None:
+            o = DeviceArray((int(max(self.dims)), rank), np.float64 if precision == PREC_F64 else np.float32)
+            keep.append(o)
+            out_ptr = o.ptr
+        res = (TuneEntry * self.nmodes)()
+        check(lib().alto_mttkrp_autotune(self.h, rank, precision, _ptr_array(list(factor_ptrs)), out_ptr, reps, stream,
+                                         C.cast(res, C.c_void_p)), "alto_mttkrp_autotune")
+        return [{k: getattr(r, k) for k, _ in TuneEntry._fields_} for r in res]
 
     def mttkrp(self, mode, factors, precision=PREC_F64):
         """mttkrp_alto_par on host arrays: returns the mode-`mode` MTTKRP."""

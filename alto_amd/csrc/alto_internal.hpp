@@ -134,6 +134,13 @@ struct alto_tensor {
     // factor rows permuted into rank order for the stream kernels, one buffer
     // per mode (kept, grown on demand)
     alto::DevBuf pcopy[ALTO_MAXN];
+
+    // MTTKRP launch shapes chosen by alto_mttkrp_autotune for (mode, rank,
+    // precision); used when the process-wide knobs are at their defaults
+    struct Tuned {
+        int mode, rank, precision, variant, seg;
+    };
+    std::vector<Tuned> tuned;
 };
 
 namespace alto {

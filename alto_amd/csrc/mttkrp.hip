@@ -201,8 +201,10 @@ int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s)
     return ALTO_OK;
 }
 
-int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* out, int precision, int accumulate,
-                  hipStream_t s, int seg)
+// variant >= -1: that lane mapping (and `seg` as given), for the autotuner;
+// -2: the process-wide knobs, else the tensor's tuned shape, else defaults
+static int mttkrp_device_shape(alto_tensor* t, int mode, int R, const void* const* U, void* out, int precision,
+                               int accumulate, hipStream_t s, int seg, int variant)
 {
     if (mode < 0 || mode >= t->nmode || R < 1 || R > 128) {
         set_error("mttkrp: mode %d / rank %d out of range (rank 1..128)", mode, R);
@@ -211,7 +213,15 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
     ALTO_TRY(mttkrp_prepare(t, R, precision, s));
     uint64_t nout = t->dims[mode] * (uint64_t)R;
     MttkrpTune k;
-    k.variant = g_variant;
+    k.variant = variant >= -1 ? variant : g_variant;
+    if (variant == -2 && seg <= 0 && g_seg == 0 && g_variant < 0) {  // shape from the autotuner, if it ran
+        for (const auto& c : t->tuned)
+            if (c.mode == mode && c.rank == R && c.precision == precision) {
+                k.variant = c.variant;
+                seg = c.seg;
+                break;
+            }
+    }
     k.nt = g_nt;
     k.hot_block = g_hot_block;
     k.hot_lds = g_hot_lds;
@@ -230,6 +240,12 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
                                   k, s);
     set_error("mttkrp: unknown precision %d", precision);
     return ALTO_ERR_ARG;
+}
+
+int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* out, int precision, int accumulate,
+                  hipStream_t s, int seg)
+{
+    return mttkrp_device_shape(t, mode, R, U, out, precision, accumulate, s, seg, -2);
 }
 
 int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s, int seg)
@@ -298,6 +314,72 @@ int alto_tune_set(int param, int value)
     }
     set_error("alto_tune_set: unknown parameter %d", param);
     return ALTO_ERR_ARG;
+}
+
+int alto_mttkrp_autotune(alto_tensor_t* t, uint64_t rank, int precision, const void* const* factors, void* out,
+                         int reps, alto_stream_t stream, alto_tune_entry_t* result)
+{
+    if (!t || !factors || !out || rank < 1 || rank > 128 || reps < 1) {
+        set_error("alto_mttkrp_autotune: invalid argument");
+        return ALTO_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int R = (int)rank;
+    static const int variants[] = {2, 0, 1};
+    static const int segs[] = {256, 512, 1024, 2048, 4096};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int st = ALTO_OK;
+    // times `reps` accumulating launches of one shape (variant -1 / seg 0:
+    // the defaults, without any tuned entry)
+    auto time_cfg = [&](int mode, int v, int sg, double* ms) -> int {
+        // warm-up first (it also prepares the streams)
+        int r = mttkrp_device_shape(t, mode, R, factors, out, precision, 1, s, sg, v);
+        if (r == ALTO_OK && hipEventRecord(e0, s) != hipSuccess) r = ALTO_ERR_HIP;
+        for (int k = 0; k < reps && r == ALTO_OK; ++k)
+            r = mttkrp_device_shape(t, mode, R, factors, out, precision, 1, s, sg, v);
+        if (r == ALTO_OK && hipEventRecord(e1, s) != hipSuccess) r = ALTO_ERR_HIP;
+        if (r == ALTO_OK && hipEventSynchronize(e1) != hipSuccess) r = ALTO_ERR_HIP;
+        float f = 0;
+        if (r == ALTO_OK && hipEventElapsedTime(&f, e0, e1) != hipSuccess) r = ALTO_ERR_HIP;
+        *ms = (double)f / reps;
+        return r;
+    };
+    // the defaults must not pick up an earlier tuned entry for this case
+    std::vector<alto_tensor::Tuned> keep;
+    for (const auto& c : t->tuned)
+        if (!(c.rank == R && c.precision == precision)) keep.push_back(c);
+    t->tuned = keep;
+    for (int mode = 0; mode < t->nmode && st == ALTO_OK; ++mode) {
+        double def_ms = 0, best_ms = 0;
+        st = time_cfg(mode, -2, 0, &def_ms);
+        int bv = -1, bs = 0;
+        best_ms = def_ms;
+        for (int v : variants)
+            for (int sg : segs) {
+                if (st != ALTO_OK) break;
+                double ms = 0;
+                st = time_cfg(mode, v, sg, &ms);
+                if (st == ALTO_OK && ms < best_ms) {
+                    best_ms = ms;
+                    bv = v;
+                    bs = sg;
+                }
+            }
+        if (st != ALTO_OK) break;
+        if (bv >= 0) t->tuned.push_back({mode, R, precision, bv, bs});
+        if (result) {
+            result[mode].mode = mode;
+            result[mode].variant = bv;
+            result[mode].seg = bs;
+            result[mode].ms = best_ms;
+            result[mode].default_ms = def_ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
 }
 
 int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* factors, void* out, int precision,

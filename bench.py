@@ -62,6 +62,7 @@ def main():
                     help="also run CP-APR on the 2^20 x 2^18 x 2^16 x 1024 Poisson count tensor (rank 10, 30 x 10)")
     ap.add_argument("--precision-study", action="store_true",
                     help="also run the 5-mode 50K x 50K x 10K x 2K x 512 Zipf(1.2) FP64 / FP32 / BF16 study (rank 64)")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default MTTKRP launch shapes")
     ap.add_argument("--mttkrp-only", type=int, default=0, metavar="REPS",
                     help="profiling driver: build the tensor, run REPS all-mode MTTKRPs, print nothing else")
     ap.add_argument("--copy-only", action="store_true",
@@ -124,6 +125,15 @@ def main():
         del sess
         return dt, fits, tm
 
+    # the library's autotuner picks the MTTKRP launch shape per mode (kept on
+    # the tensor; the CP-ALS sessions below use it)
+    tuned = {}
+    if not args.no_autotune:
+        t0 = time.time()
+        tuned["fp64"] = T.mttkrp_autotune(RANK, A.PREC_F64, reps=3, stream=stream)
+        log(f"autotune fp64 ({time.time() - t0:.1f}s): " +
+            ", ".join(f"mode {e['mode']}: v{e['variant']} seg {e['seg']} {e['ms']:.3f} ms (default {e['default_ms']:.3f})"
+                      for e in tuned["fp64"]))
     dt, fits, tm = run_als(A.PREC_F64, args.steps, args.warmup)
     ms_per_step = dt / args.steps * 1e3
     log(f"FP64 CP-ALS: {args.steps} iterations in {dt:.3f}s ({ms_per_step:.3f} ms/iter), fit {fits[-1]:.6f}")
@@ -153,6 +163,8 @@ def main():
     variants = {}
     if not args.no_variants:
         for name, prec in (("fp32", A.PREC_F32), ("bf16", A.PREC_BF16)):
+            if not args.no_autotune:
+                tuned[name] = T.mttkrp_autotune(RANK, prec, reps=3, stream=stream)
             vdt, vfits, _ = run_als(prec, args.steps, args.warmup)
             variants[name] = {"iters_per_s": args.steps / vdt, "ms_per_step": vdt / args.steps * 1e3,
                               "fit": float(vfits[-1]), "fit_delta_vs_fp64": float(abs(vfits[-1] - fits[-1]))}
@@ -189,6 +201,7 @@ def main():
                      "kernel": "k_mttkrp16 (all-mode, 3 launches)", "ms_per_launch": mode_ms,
                      "algorithmic_bytes_per_launch": mode_bytes},
         "mttkrp_allmode_ms": allmode_ms,
+        "autotune": tuned,
         "alto_create_s": create_s,
         "cpd_timers_s": tm,
         "precision_variants": variants,

@@ -136,6 +136,27 @@ enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, A
        ALTO_TUNE_NARROW_RANK = 10 };
 int alto_tune_set(int param, int value);
 
+/* Autotuner (no reference counterpart; the reference fixes its shapes at
+ * compile time, config.mk:25-26): for every mode of `t`, times `reps`
+ * MTTKRPs of rank `rank` in `precision` for each lane mapping (0, 1, 2) and
+ * segment length (256 .. 4096) against the defaults, and keeps the fastest
+ * shape on the tensor; later MTTKRPs of that (mode, rank, precision) on `t`
+ * -- CP-ALS sessions included -- use it while ALTO_TUNE_SEG and
+ * ALTO_TUNE_MTTKRP_VARIANT are at their defaults.  `factors` are device
+ * factor matrices of that precision; `out` is device scratch of
+ * max_n dims[n] x rank elements of the output type (FP64, or FP32 for FP32
+ * and BF16), overwritten.  `result` (may be NULL) receives nmode entries
+ * (variant -1 / seg 0: the defaults were fastest).  Synchronous. */
+typedef struct {
+    int mode;
+    int variant;
+    int seg;
+    double ms;          /* mean time of the chosen shape */
+    double default_ms;  /* mean time with the defaults */
+} alto_tune_entry_t;
+int alto_mttkrp_autotune(alto_tensor_t* t, uint64_t rank, int precision, const void* const* factors, void* out,
+                         int reps, alto_stream_t stream, alto_tune_entry_t* result);
+
 /* ---- CP-ALS --------------------------------------------------------------- */
 
 typedef struct {

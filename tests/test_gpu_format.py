@@ -179,3 +179,32 @@ def test_mttkrp_tuning_knobs_keep_results(tune_reset, hot, variant, chunk_log2, 
             assert _rel(g.mttkrp(mode, U, A.PREC_F32).astype(np.float64), want) < 1e-4
             Ub = [A.from_bf16_bits(A.to_bf16_bits(u)).astype(np.float64) for u in U]
             assert _rel(g.mttkrp(mode, U, A.PREC_BF16).astype(np.float64), mttkrp_numpy(cidx, vals, Ub, mode)) < 1e-4
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_mttkrp_autotune_keeps_results(prec):
+    """The autotuner times every lane mapping and segment length per mode,
+    keeps the fastest on the tensor, and the MTTKRPs that then use it still
+    match the oracle (FP64 1e-12; FP32 / BF16 1e-4 of the FP64 result on
+    the same stored factors)."""
+    A = _alto()
+    dims = [300, 400, 2000]
+    cidx, vals = zipf_tensor(dims, 60000, 23)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    o = oracle.OracleAlto(dims, cidx, vals, W=1, nprtn=8)
+    R = 32
+    res = g.mttkrp_autotune(R, prec, reps=2)
+    assert [r["mode"] for r in res] == [0, 1, 2]
+    for r in res:
+        assert r["variant"] in (-1, 0, 1, 2) and r["seg"] in (0, 256, 512, 1024, 2048, 4096)
+        assert r["ms"] <= r["default_ms"] + 1e-12
+    rng = np.random.default_rng(4)
+    U = [rng.random((d, R)) for d in dims]
+    for mode in range(3):
+        if prec == 0:
+            want = o.mttkrp(mode, U)
+            np.testing.assert_allclose(g.mttkrp(mode, U), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+        else:
+            Us = U if prec == 1 else [A.from_bf16_bits(A.to_bf16_bits(u)).astype(np.float64) for u in U]
+            Us = [u.astype(np.float32).astype(np.float64) for u in Us]
+            assert _rel(g.mttkrp(mode, U, prec).astype(np.float64), mttkrp_numpy(cidx, vals, Us, mode)) < 1e-4
