@@ -78,20 +78,23 @@ def test_pseudo_inverse_fallback_matches_gelsy():
     np.testing.assert_allclose(out, want, rtol=1e-8, atol=1e-10)
 
 
-def _als(dims, cidx, vals, R, iters, prec=0, seed=3, eps=0.0):
+def _als(dims, cidx, vals, R, iters, prec=0, seed=3, eps=0.0, lit_bits=0):
     A = _A()
-    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8, lit_bits=lit_bits)
     U, lam = oracle.random_init(dims, R, seed)
     n, fit, tm = g.cpd_als(U, lam, iters, eps, prec)
     return n, fit, U, lam
 
 
-@pytest.mark.parametrize("dims,nnz,R", [([40, 50, 60], 6000, 8), ([30, 20, 10, 40], 8000, 16),
-                                        ([100, 90, 80, 70, 60], 20000, 10), ([200, 300, 250], 30000, 32)])
-def test_cpd_als_fp64_matches_oracle(dims, nnz, R):
+@pytest.mark.parametrize("dims,nnz,R,W", [([40, 50, 60], 6000, 8, 1), ([30, 20, 10, 40], 8000, 16, 1),
+                                          ([100, 90, 80, 70, 60], 20000, 10, 1), ([200, 300, 250], 30000, 32, 1),
+                                          ([2**14, 2**13, 2**13, 2**12, 2**12], 20000, 16, 2),
+                                          ([300, 400, 500], 20000, 64, 2)])
+def test_cpd_als_fp64_matches_oracle(dims, nnz, R, W):
+    """W = 2: 128-bit ALTO keys (66 index bits, and a forced 128-bit mask)."""
     cidx, vals = uniform_tensor(dims, nnz, 5)
-    n, fit, U, lam = _als(dims, cidx, vals, R, 12)
-    o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
+    n, fit, U, lam = _als(dims, cidx, vals, R, 12, lit_bits=64 * W)
+    o = oracle.OracleAlto(dims, cidx, vals, W=W, nprtn=8)
     Uo, lo = oracle.random_init(dims, R, 3)
     no, fo = o.cpd_als(Uo, lo, 12, 0.0)
     assert n == no == 12
