@@ -158,4 +158,7 @@ int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStr
 int split_stream(alto_tensor* t, int mode, const int* hot_k, hipStream_t s);
 int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s);
 
+// cpd.hip: per-phase event timing of a session's eager iterations (default on)
+int cpd_session_set_timing(alto_cpd_session_t* sess, int on);
+
 }  // namespace alto

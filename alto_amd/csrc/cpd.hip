@@ -76,7 +76,7 @@ template <typename UT, typename ST, typename MT>
 struct CpdState {
     int N = 0, R = 0, prec = 0;
     DevBuf U[ALTO_MAXN], S[ALTO_MAXN];
-    DevBuf M, scratch, grams, gpart, L, P, wsA, wsQ, status, psum, pmax, lambda, ipart, fitbuf, nx, nxpart;
+    DevBuf M, scratch, grams, gpart, L, P, wsA, wsQ, status, psum, pmax, pin, lambda, ipart, fitbuf, nx, nxpart;
     int nb_solve = 1;
     UT* u(int m) { return U[m].as<UT>(); }
     ST* st(int m) { return std::is_same<UT, ST>::value ? reinterpret_cast<ST*>(U[m].p) : S[m].as<ST>(); }
@@ -96,7 +96,6 @@ static int cpd_alloc(alto_tensor* t, int R, CpdState<UT, ST, MT>& c)
     maxrows_solve = maxd;
     c.nb_solve = solve_blocks(maxrows_solve, R);
     ALTO_TRY(c.M.alloc(maxd * R * sizeof(MT)));
-    ALTO_TRY(c.scratch.alloc(t->dims[t->nmode - 1] * R * sizeof(MT)));
     ALTO_TRY(c.grams.alloc((size_t)t->nmode * R * R * 8));
     ALTO_TRY(c.gpart.alloc((size_t)gram_blocks(maxd) * R * R * 8));
     ALTO_TRY(c.L.alloc((size_t)R * R * 8));
@@ -106,6 +105,7 @@ static int cpd_alloc(alto_tensor* t, int R, CpdState<UT, ST, MT>& c)
     ALTO_TRY(c.status.alloc(8));
     ALTO_TRY(c.psum.alloc((size_t)c.nb_solve * R * 8));
     ALTO_TRY(c.pmax.alloc((size_t)c.nb_solve * R * 8));
+    ALTO_TRY(c.pin.alloc((size_t)c.nb_solve * 8));
     ALTO_TRY(c.lambda.alloc((size_t)R * 8));
     ALTO_TRY(c.ipart.alloc((size_t)64 * R * 8));
     ALTO_TRY(c.fitbuf.alloc(4 * 8));
@@ -186,35 +186,32 @@ static int cpd_mode_step(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it,
     ALTO_TRY(pt.mark(-1));
     ALTO_TRY(mttkrp_device(t, j, R, fptr, c.M.p, c.prec, 0, s, 0));
     ALTO_TRY(pt.mark(PH_MTTKRP));
-    if (j == N - 1)
-        HIP_TRY(hipMemcpyAsync(c.scratch.p, c.M.p, t->dims[j] * R * sizeof(MT), hipMemcpyDeviceToDevice, s));
-    ALTO_TRY(pt.mark(PH_COPY));
     ALTO_TRY(chol_device(c.grams.template as<double>(), N, j, R, c.L.template as<double>(), c.P.template as<double>(),
                          c.wsA.template as<double>(), c.wsQ.template as<double>(), c.status.template as<int>(), s));
+    // the last mode's solve also leaves the fit's inner-product partials
     ALTO_TRY((solve_device<MT, UT>(c.M.template as<MT>(), t->dims[j], R, c.L.template as<double>(),
                                    c.P.template as<double>(), c.status.template as<int>(), c.u(j),
-                                   c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, s)));
+                                   c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, s,
+                                   j == N - 1 ? c.pin.template as<double>() : nullptr)));
     ALTO_TRY(pt.mark(PH_PINV));
     ALTO_TRY(lambda_device(c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, R, it == 0 ? 0 : 1,
                            c.lambda.template as<double>(), s));
-    ALTO_TRY((scale_device<UT, ST>(c.u(j), t->dims[j], R, c.lambda.template as<double>(),
-                                   std::is_same<UT, ST>::value ? nullptr : c.st(j), s)));
     ALTO_TRY(pt.mark(PH_NORM));
-    ALTO_TRY(gram_device<ST>(c.st(j), t->dims[j], R, c.grams.template as<double>() + (size_t)j * R * R,
-                             c.gpart.template as<double>(), s, gram_f64(c.prec)));
+    ALTO_TRY((gram_scale_device<UT, ST>(c.u(j), t->dims[j], R, c.lambda.template as<double>(),
+                                        std::is_same<UT, ST>::value ? nullptr : c.st(j),
+                                        c.grams.template as<double>() + (size_t)j * R * R,
+                                        c.gpart.template as<double>(), s, gram_f64(c.prec))));
     ALTO_TRY(pt.mark(PH_UPDATE));
     return ALTO_OK;
 }
 
+// fit after an iteration: the inner product came out of the last mode's solve
 template <typename UT, typename ST, typename MT>
 static int cpd_fit_step(alto_tensor* t, CpdState<UT, ST, MT>& c, double normX, hipStream_t s)
 {
-    const int N = c.N, R = c.R;
-    ALTO_TRY((inner_partial_device<UT, MT>(c.u(N - 1), c.scratch.template as<MT>(), t->dims[N - 1], R,
-                                           c.ipart.template as<double>(), s)));
-    ALTO_TRY(fit_device(c.grams.template as<double>(), N, R, c.lambda.template as<double>(), c.ipart.template as<double>(),
-                        normX, c.fitbuf.template as<double>(), s));
-    return ALTO_OK;
+    (void)t;
+    return fit_scalar_device(c.grams.template as<double>(), c.N, c.R, c.lambda.template as<double>(),
+                             c.pin.template as<double>(), c.nb_solve, normX, c.fitbuf.template as<double>(), s);
 }
 
 // A CP-ALS model resident on the device (factors, Grams, workspaces) that
@@ -225,6 +222,9 @@ struct CpdSessionBase {
     int R = 0, prec = 0, it = 0;
     double prev_fit = 0, fit = 0, normX = 0;
     bool converged = false;
+    // per-phase event timing of the eager iterations (reported by
+    // alto_cpd_session_get); off for callers that do not read it
+    bool timed = true;
     hipStream_t s = nullptr;
     double acc[PH_N] = {0};
     double total = 0;
@@ -310,7 +310,7 @@ struct CpdSession : CpdSessionBase {
     {
         if (graphs_enabled() && !verbose) return iterate_graph(max_iters, eps, fit_trace, ran);
         PhaseTimer pt;
-        ALTO_TRY(pt.init(true, s));
+        ALTO_TRY(pt.init(timed || verbose, s));
         int done = 0;
         const double t0 = now_s();
         for (int k = 0; k < max_iters && !converged; ++k, ++it) {
@@ -463,11 +463,16 @@ int cpd_fit_impl(alto_tensor* t, int R, double* const* Uh, const double* lambda_
     HIP_TRY(hipMemcpy(c.lambda.p, lambda_h, R * 8, hipMemcpyHostToDevice));
     const void* fptr[ALTO_MAXN];
     for (int m = 0; m < c.N; ++m) fptr[m] = c.st(m);
+    ALTO_TRY(c.scratch.alloc(t->dims[c.N - 1] * R * 8));
     ALTO_TRY(mttkrp_device(t, c.N - 1, R, fptr, c.scratch.p, ALTO_PREC_F64, 0, s, 0));
     ALTO_TRY(normsq_device(t->vals.as<double>(), t->nnz, c.nxpart.template as<double>(), c.nx.template as<double>(), s));
     double normX = 0;
     HIP_TRY(hipMemcpy(&normX, c.nx.p, 8, hipMemcpyDeviceToHost));
-    ALTO_TRY(cpd_fit_step(t, c, normX, s));
+    // the model is given, so the inner product comes from its own partials
+    ALTO_TRY((inner_partial_device<double, double>(c.u(c.N - 1), c.scratch.as<double>(), t->dims[c.N - 1], R,
+                                                   c.ipart.as<double>(), s)));
+    ALTO_TRY(fit_device(c.grams.as<double>(), c.N, R, c.lambda.as<double>(), c.ipart.as<double>(), normX,
+                        c.fitbuf.as<double>(), s));
     double fb[3];
     HIP_TRY(hipMemcpy(fb, c.fitbuf.p, 24, hipMemcpyDeviceToHost));
     *fit = fb[0];
@@ -475,6 +480,13 @@ int cpd_fit_impl(alto_tensor* t, int R, double* const* Uh, const double* lambda_
 }
 
 }  // namespace alto
+
+int alto::cpd_session_set_timing(alto_cpd_session_t* sess, int on)
+{
+    if (!sess) return ALTO_ERR_ARG;
+    reinterpret_cast<alto::CpdSessionBase*>(sess)->timed = on != 0;
+    return ALTO_OK;
+}
 
 using namespace alto;
 

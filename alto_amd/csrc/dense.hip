@@ -91,9 +91,15 @@ __global__ void __launch_bounds__(256) k_gram_partial(const T* __restrict__ U, u
 // U[i0 + l/16][16 cb + l%16]) and multiplies every block pair ta <= tb into
 // its accumulators (the Gram is symmetric); the 4 waves' tiles are summed in
 // LDS and the block writes both triangles of its partial.
-template <typename T, bool F64, int T16>
-__global__ void __launch_bounds__(256) k_gram_rows(const T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
-                                                   double* __restrict__ part)
+//
+// SCALE: the block first divides its rows of U by lambda (per column) in
+// place and writes the storage copy S (when the storage type differs), and
+// the Gram is taken of the stored values — the column normalisation and the
+// Gram update of a mode in one pass over U.
+template <typename T, typename ST, bool F64, int T16, bool SCALE>
+__global__ void __launch_bounds__(256) k_gram_rows(T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
+                                                   double* __restrict__ part, const double* __restrict__ lambda,
+                                                   ST* __restrict__ S)
 {
     constexpr int NP = T16 * (T16 + 1) / 2;
     using acc_t = typename std::conditional<F64, d4_t, f4_t>::type;
@@ -105,24 +111,58 @@ __global__ void __launch_bounds__(256) k_gram_rows(const T* __restrict__ U, uint
     for (int q = 0; q < NP; ++q) acc[q] = acc_t{0, 0, 0, 0};
     const uint64_t base = (uint64_t)blockIdx.x * rows_per_block;
     const uint64_t end = base + rows_per_block < I ? base + rows_per_block : I;
-    for (uint64_t i0 = base + 4 * wave; i0 < end; i0 += 16) {
-        const uint64_t i = i0 + kq;
-        double x[T16];
+    double lc[T16];
+    if constexpr (SCALE) {
 #pragma unroll
-        for (int cb = 0; cb < T16; ++cb) {
-            const int c = cb * 16 + r16;
-            x[cb] = (i < end && c < R) ? as_f64(U[i * R + c]) : 0.0;
-        }
-        int q = 0;
+        for (int cb = 0; cb < T16; ++cb) lc[cb] = cb * 16 + r16 < R ? lambda[cb * 16 + r16] : 1.0;
+    }
+    // two 16-row steps per pass: both steps' loads are issued before the products
+    for (uint64_t i00 = base + 4 * wave; i00 < end; i00 += 32) {
+        double raw[2][T16];
 #pragma unroll
-        for (int ta = 0; ta < T16; ++ta)
+        for (int u = 0; u < 2; ++u) {
+            const uint64_t i = i00 + 16 * u + kq;
 #pragma unroll
-            for (int tb = ta; tb < T16; ++tb, ++q) {
-                if constexpr (F64)
-                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[ta], x[tb], acc[q], 0, 0, 0);
-                else
-                    acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)x[ta], (float)x[tb], acc[q], 0, 0, 0);
+            for (int cb = 0; cb < T16; ++cb) {
+                const int c = cb * 16 + r16;
+                raw[u][cb] = (i < end && c < R) ? as_f64(U[i * R + c]) : 0.0;
             }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint64_t i = i00 + 16 * u + kq;
+            double x[T16];
+#pragma unroll
+            for (int cb = 0; cb < T16; ++cb) {
+                const int c = cb * 16 + r16;
+                if constexpr (SCALE) {
+                    x[cb] = 0.0;
+                    if (i < end && c < R) {
+                        const double v = raw[u][cb] / lc[cb];
+                        U[i * R + c] = (T)v;
+                        if constexpr (std::is_same<T, ST>::value) {
+                            x[cb] = (double)(T)v;
+                        } else {
+                            const ST sv = store_as<ST>(v);
+                            S[i * R + c] = sv;
+                            x[cb] = as_f64(sv);
+                        }
+                    }
+                } else {
+                    x[cb] = raw[u][cb];
+                }
+            }
+            int q = 0;
+#pragma unroll
+            for (int ta = 0; ta < T16; ++ta)
+#pragma unroll
+                for (int tb = ta; tb < T16; ++tb, ++q) {
+                    if constexpr (F64)
+                        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[ta], x[tb], acc[q], 0, 0, 0);
+                    else
+                        acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32((float)x[ta], (float)x[tb], acc[q], 0, 0, 0);
+                }
+        }
     }
     for (int k = threadIdx.x; k < R * R; k += blockDim.x) sG[k] = 0.0;
     __syncthreads();
@@ -162,6 +202,28 @@ __global__ void k_sum_partials(const double* __restrict__ part, int nparts, int 
     }
 }
 
+// out[k] = sum_p part[p n + k]: a block covers 32 consecutive k, its 8
+// wave halves stride over the partials, and the 8 sums are added in a fixed
+// order (deterministic)
+__global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ part, int nparts, int n,
+                                                  double* __restrict__ out)
+{
+    __shared__ double s[8][33];
+    const int kk = threadIdx.x & 31, pg = threadIdx.x >> 5;
+    const int k = blockIdx.x * 32 + kk;
+    double a = 0;
+    if (k < n)
+        for (int p = pg; p < nparts; p += 8) a += part[(size_t)p * n + k];
+    s[pg][kk] = a;
+    __syncthreads();
+    if (threadIdx.x < 32 && k < n) {
+        double v = 0;
+#pragma unroll
+        for (int g = 0; g < 8; ++g) v += s[g][kk];
+        out[k] = v;
+    }
+}
+
 int gram_blocks(uint64_t I)
 {
     uint64_t b = (I + GRAM_ROWS - 1) / GRAM_ROWS;
@@ -180,10 +242,15 @@ int gram_device(const T* U, uint64_t I, int R, double* G, double* part, hipStrea
     if (R <= 64) {
         const uint64_t rpb = ((I + nb - 1) / nb + 15) / 16 * 16;
         const size_t lds = (size_t)R * R * 8;
-#define GR_(TT)                                                                                                      \
-    do {                                                                                                             \
-        if (f64) hipLaunchKernelGGL((k_gram_rows<T, true, TT>), dim3(nb), dim3(256), lds, s, U, I, R, rpb, part);  \
-        else hipLaunchKernelGGL((k_gram_rows<T, false, TT>), dim3(nb), dim3(256), lds, s, U, I, R, rpb, part);     \
+    T* Uw = const_cast<T*>(U);  // read only without SCALE
+#define GR_(TT)                                                                                                  \
+    do {                                                                                                         \
+        if (f64)                                                                                                 \
+            hipLaunchKernelGGL((k_gram_rows<T, T, true, TT, false>), dim3(nb), dim3(256), lds, s, Uw, I, R, rpb, \
+                               part, nullptr, nullptr);                                                          \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_gram_rows<T, T, false, TT, false>), dim3(nb), dim3(256), lds, s, Uw, I, R,     \
+                               rpb, part, nullptr, nullptr);                                                     \
     } while (0)
         switch ((R + 15) / 16) {
         case 1: GR_(1); break;
@@ -197,77 +264,156 @@ int gram_device(const T* U, uint64_t I, int R, double* G, double* part, hipStrea
     } else {
         hipLaunchKernelGGL((k_gram_partial<T, false>), dim3(nb), dim3(256), 0, s, U, I, R, part);
     }
-    hipLaunchKernelGGL(k_sum_partials, dim3((R * R + 255) / 256), dim3(256), 0, s, part, nb, R * R, G);
+    hipLaunchKernelGGL(k_sum_cols, dim3((R * R + 31) / 32), dim3(256), 0, s, part, nb, R * R, G);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }
+template <typename UT, typename ST>
+int gram_scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, double* G, double* part, hipStream_t s,
+                      bool f64)
+{
+    constexpr bool same = std::is_same<UT, ST>::value;
+    if (R > 64) {
+        ALTO_TRY((scale_device<UT, ST>(U, I, R, lambda, same ? nullptr : S, s)));
+        return gram_device<ST>(same ? reinterpret_cast<const ST*>(U) : S, I, R, G, part, s, f64);
+    }
+    const int nb = gram_blocks(I);
+    const uint64_t rpb = ((I + nb - 1) / nb + 15) / 16 * 16;
+    const size_t lds = (size_t)R * R * 8;
+#define GS_(TT)                                                                                                 \
+    do {                                                                                                        \
+        if (f64)                                                                                                \
+            hipLaunchKernelGGL((k_gram_rows<UT, ST, true, TT, true>), dim3(nb), dim3(256), lds, s, U, I, R, rpb, \
+                               part, lambda, S);                                                                \
+        else                                                                                                    \
+            hipLaunchKernelGGL((k_gram_rows<UT, ST, false, TT, true>), dim3(nb), dim3(256), lds, s, U, I, R,    \
+                               rpb, part, lambda, S);                                                           \
+    } while (0)
+    switch ((R + 15) / 16) {
+    case 1: GS_(1); break;
+    case 2: GS_(2); break;
+    case 3: GS_(3); break;
+    default: GS_(4); break;
+    }
+#undef GS_
+    hipLaunchKernelGGL(k_sum_cols, dim3((R * R + 31) / 32), dim3(256), 0, s, part, nb, R * R, G);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+template int gram_scale_device<double, double>(double*, uint64_t, int, const double*, double*, double*, double*,
+                                               hipStream_t, bool);
+template int gram_scale_device<float, float>(float*, uint64_t, int, const double*, float*, double*, double*,
+                                             hipStream_t, bool);
+template int gram_scale_device<float, bf16_t>(float*, uint64_t, int, const double*, bf16_t*, double*, double*,
+                                              hipStream_t, bool);
+
 template int gram_device<double>(const double*, uint64_t, int, double*, double*, hipStream_t, bool);
 template int gram_device<float>(const float*, uint64_t, int, double*, double*, hipStream_t, bool);
 template int gram_device<bf16_t>(const bf16_t*, uint64_t, int, double*, double*, hipStream_t, bool);
 
 // ------------------------------------------------------------------ V and its Cholesky factor
 // V^-1 for symmetric positive definite V = prod_{m != mode} G_m by in-place
-// Gauss-Jordan elimination without pivoting in LDS (one workgroup, R <= 128):
-// the k-th pivot is the k-th Schur complement diagonal, i.e. the square of
-// the Cholesky diagonal, so a pivot <= 0 is exactly the Cholesky failure of
-// the reference's dpotrf test; then status = 1 and the pseudo-inverse path
-// (k_pinv) takes over.  status 0: W = V^-1.
-__global__ void __launch_bounds__(256) k_spd_inverse(const double* __restrict__ grams, int nmode, int mode, int R,
-                                                     double* __restrict__ W, int* __restrict__ status)
+// Gauss-Jordan elimination without pivoting (one workgroup, R <= 128): the
+// k-th pivot is the k-th Schur complement diagonal, i.e. the square of the
+// Cholesky diagonal, so a pivot <= 0 is exactly the Cholesky failure of the
+// reference's dpotrf test; then status = 1 and the same workgroup computes
+// the pseudo-inverse (pinv_body).  status 0: W = V^-1.
+//
+// Thread t keeps the entries (r0 + rstep n, jc) of the matrix in registers,
+// rstep = BT / R, r0 = t / R, jc = t % R (BT threads: 1024 from R = 17 on,
+// so that a thread holds at most 16 entries and the waves of all four SIMDs
+// share each step).  Step k publishes column k (the
+// elimination factors, pivot included) and row k through LDS and updates
+// every register; the two LDS buffers alternate between steps, so one
+// barrier per step orders them (a thread writes buffer k & 1 at step k only
+// after the barrier of step k - 1, which every reader of step k - 2 passed).
+__device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ grams, int nmode, int mode, int R,
+                                                    double* __restrict__ A, double* __restrict__ Q,
+                                                    double* __restrict__ Pout);
+
+template <int BT, int NPT>
+__global__ void __launch_bounds__(BT) k_spd_inverse(const double* __restrict__ grams, int nmode, int mode, int R,
+                                                    double* __restrict__ W, int* __restrict__ status,
+                                                    double* __restrict__ wsA, double* __restrict__ wsQ)
 {
-    extern __shared__ double A[];  // R x R, row stride R
-    __shared__ double s_col[128];
-    __shared__ int ok;
+    __shared__ double s_col[2][128], s_row[2][128];
     const int tid = threadIdx.x;
-    for (int k = tid; k < R * R; k += blockDim.x) {
-        double v = 1.0;
-        for (int m = 0; m < nmode; ++m)
-            if (m != mode) v *= grams[(size_t)m * R * R + k];
-        A[k] = v;
-    }
-    if (tid == 0) ok = 1;
-    __syncthreads();
-    // elimination layout: thread -> column jc, rows tid / R + rstep * n
-    const int rstep = (int)blockDim.x / R, jc = tid % R;
-    for (int k = 0; k < R; ++k) {
-        const double piv = A[k * R + k];
-        if (!(piv > 0.0)) {  // uniform: every thread read the same pivot
-            if (tid == 0) ok = 0;
-            break;
+    const int rstep = BT / R, r0 = tid / R, jc = tid % R;
+    const bool active = tid < rstep * R;
+    // rows of this thread are clamped into the matrix so that every load and
+    // LDS read below is unconditional (entries of rows >= R are never stored)
+    auto rc = [&](int n) { return min(r0 + rstep * n, R - 1); };
+    double a[NPT];
+#pragma unroll
+    for (int n = 0; n < NPT; ++n) a[n] = 1.0;
+#pragma unroll
+    for (int m = 0; m < ALTO_MAXN; ++m) {
+        if (m < nmode && m != mode) {
+            const double* g = grams + (size_t)m * R * R + (active ? jc : 0);
+#pragma unroll
+            for (int n = 0; n < NPT; ++n) a[n] *= g[rc(n) * R];
         }
-        // column k of the other rows (the elimination factors), then row k
-        // scaled by 1 / pivot with its diagonal slot holding 1 / pivot
-        for (int i = tid; i < R; i += blockDim.x) s_col[i] = i == k ? 0.0 : A[i * R + k];
-        __syncthreads();
-        for (int j = tid; j < R; j += blockDim.x) A[k * R + j] = (j == k ? 1.0 : A[k * R + j]) / piv;
-        __syncthreads();
-        if (tid < rstep * R) {
-            const double akj = A[k * R + jc];
-            for (int i = tid / R; i < R; i += rstep) {
-                if (i == k) continue;
-                double* a = &A[i * R + jc];
-                *a = (jc == k ? 0.0 : *a) - s_col[i] * akj;
+    }
+    // the step at which this thread publishes row k: k = r0 + rstep * n
+    int ok = 1;
+    for (int k = 0; k < R; ++k) {
+        const int b = k & 1;
+        if (active) {
+            if (jc == k) {
+#pragma unroll
+                for (int n = 0; n < NPT; ++n)
+                    if (r0 + rstep * n < R) s_col[b][r0 + rstep * n] = a[n];
+            }
+            if (k >= r0 && (k - r0) % rstep == 0) {
+                double rv = 0.0;
+#pragma unroll
+                for (int n = 0; n < NPT; ++n) rv = r0 + rstep * n == k ? a[n] : rv;
+                s_row[b][jc] = rv;
             }
         }
         __syncthreads();
+        const double piv = s_col[b][k];
+        if (!(piv > 0.0)) {  // uniform: every thread read the same pivot
+            ok = 0;
+            break;
+        }
+        // row k scaled by 1 / pivot, its diagonal slot holding 1 / pivot;
+        // the other rows lose s_col[i] times it (column k becomes -s_col / piv)
+        const double rk = (jc == k ? 1.0 : s_row[b][jc]) / piv;
+        constexpr int CH = NPT < 16 ? NPT : 16;  // LDS reads in flight per batch
+#pragma unroll
+        for (int n0 = 0; n0 < NPT; n0 += CH) {
+            double cv[CH];
+#pragma unroll
+            for (int n = 0; n < CH; ++n) cv[n] = s_col[b][rc(n0 + n)];
+#pragma unroll
+            for (int n = 0; n < CH; ++n)
+                a[n0 + n] = r0 + rstep * (n0 + n) == k ? rk : (jc == k ? 0.0 : a[n0 + n]) - cv[n] * rk;
+        }
     }
-    __syncthreads();
     if (ok) {
-        for (int e = tid; e < R * R; e += blockDim.x) W[e] = A[e];
+        if (active) {
+#pragma unroll
+            for (int n = 0; n < NPT; ++n)
+                if (r0 + rstep * n < R) W[(r0 + rstep * n) * R + jc] = a[n];
+        }
+        if (tid == 0) *status = 0;
+        return;
     }
-    if (tid == 0) *status = ok ? 0 : 1;
+    if (tid == 0) *status = 1;
+    __syncthreads();
+    pinv_body(grams, nmode, mode, R, wsA, wsQ, W);
 }
 
 // Minimum-norm pseudo-inverse V^+ = Q diag(1/e) Q^T over |e| > R*eps*max|e|
 // (cyclic Jacobi; runs only after a failed Cholesky).  A and Q live in the
 // workspace (global memory), shared through work-group barriers.
-__global__ void __launch_bounds__(256) k_pinv(const double* __restrict__ grams, int nmode, int mode, int R,
-                                              double* __restrict__ A, double* __restrict__ Q, double* __restrict__ Pout,
-                                              const int* __restrict__ status)
+__device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ grams, int nmode, int mode, int R,
+                                                    double* __restrict__ A, double* __restrict__ Q,
+                                                    double* __restrict__ Pout)
 {
-    if (*status == 0) return;
     const int tid = threadIdx.x;
-    __shared__ double s_red[256];
+    __shared__ double s_red[1024];
     __shared__ double cs[2];
     __shared__ int s_stop;
     for (int k = tid; k < R * R; k += blockDim.x) {
@@ -300,7 +446,7 @@ __global__ void __launch_bounds__(256) k_pinv(const double* __restrict__ grams, 
         }
         s_red[tid] = off;
         __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
+        for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
             if (tid < o) s_red[tid] += s_red[tid + o];
             __syncthreads();
         }
@@ -308,7 +454,7 @@ __global__ void __launch_bounds__(256) k_pinv(const double* __restrict__ grams, 
         __syncthreads();
         s_red[tid] = tot;
         __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
+        for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
             if (tid < o) s_red[tid] += s_red[tid + o];
             __syncthreads();
         }
@@ -371,74 +517,83 @@ __global__ void __launch_bounds__(256) k_pinv(const double* __restrict__ grams, 
 // [k0 + l/16]) and multiplies it into every 16-column tile of the output
 // (B operand from the LDS copy of W: lane l holds W[k0 + l/16][c0 + l%16]).
 // Fused with the per-block column sums of squares / maxima of the stored
-// values (for lambda).
-constexpr int APPLY_MAXT = 8;  // 16-column tiles (R <= 128)
-
-template <typename MT, typename UT>
+// values (for lambda) and, when pin is given, the per-block sum of
+// U[i,c] M[i,c] over the block's rows (the fit's inner product: with
+// U = lambda * U_normalised it equals sum_c lambda_c <U_normalised[:,c], M[:,c]>).
+// The 16 x R block of M is loaded into registers before the products, so
+// its loads are in flight together.  T16: 16-column tiles (R <= 16 T16).
+template <typename MT, typename UT, int T16>
 __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ Wg,
-                                               UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax)
+                                               UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
+                                               double* __restrict__ pin)
 {
+    constexpr int KS = 4 * T16;  // k-steps of 4 covering 16 T16 >= R columns
     extern __shared__ double sW[];  // W, row stride WS
     const int WS = R + 2;
     for (int k = threadIdx.x; k < R * R; k += blockDim.x) sW[(k / R) * WS + k % R] = Wg[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, kq = lane >> 4;
-    const int T16 = (R + 15) / 16, KS = (R + 3) / 4;
-    double ss[APPLY_MAXT], mx[APPLY_MAXT];
+    double ss[T16], mx[T16];
 #pragma unroll
-    for (int t = 0; t < APPLY_MAXT; ++t) {
+    for (int t = 0; t < T16; ++t) {
         ss[t] = 0;
         mx[t] = -INFINITY;
     }
+    double in = 0;
     for (uint64_t row0 = ((uint64_t)blockIdx.x * 4 + wave) * 16; row0 < I; row0 += (uint64_t)gridDim.x * 64) {
-        d4_t acc[APPLY_MAXT];
-#pragma unroll
-        for (int t = 0; t < APPLY_MAXT; ++t) acc[t] = d4_t{0, 0, 0, 0};
         const uint64_t ra = row0 + r16;
+        double av[KS];
+#pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int k = ks * 4 + kq;
-            const double a = (ra < I && k < R) ? (double)M[ra * R + k] : 0.0;
+            av[ks] = (ra < I && k < R) ? (double)M[ra * R + k] : 0.0;
+        }
+        d4_t acc[T16];
 #pragma unroll
-            for (int t = 0; t < APPLY_MAXT; ++t) {
-                if (t < T16) {
+        for (int t = 0; t < T16; ++t) acc[t] = d4_t{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            if (ks * 4 < R) {  // uniform
+                const int k = ks * 4 + kq;
+#pragma unroll
+                for (int t = 0; t < T16; ++t) {
                     const int c = t * 16 + r16;
                     const double b = (k < R && c < R) ? sW[k * WS + c] : 0.0;
-                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], b, acc[t], 0, 0, 0);
                 }
             }
         }
         // register v of lane l holds C[l/16 + 4v][l%16]
 #pragma unroll
-        for (int t = 0; t < APPLY_MAXT; ++t) {
-            if (t < T16) {
-                const int c = t * 16 + r16;
+        for (int t = 0; t < T16; ++t) {
+            const int c = t * 16 + r16;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const uint64_t r = row0 + kq + 4 * v;
-                    if (r < I && c < R) {
-                        const UT x = (UT)acc[t][v];
-                        U[r * R + c] = x;
-                        const double xd = (double)x;
-                        ss[t] += xd * xd;
-                        mx[t] = fmax(mx[t], xd);
-                    }
+            for (int v = 0; v < 4; ++v) {
+                const uint64_t r = row0 + kq + 4 * v;
+                if (r < I && c < R) {
+                    const UT x = (UT)acc[t][v];
+                    U[r * R + c] = x;
+                    const double xd = (double)x;
+                    ss[t] += xd * xd;
+                    mx[t] = fmax(mx[t], xd);
+                    if (pin) in += xd * (double)M[r * R + c];
                 }
             }
         }
     }
     // column statistics: lanes with equal l%16 hold the same column
-    __shared__ double s_ss[4 * 16 * APPLY_MAXT], s_mx[4 * 16 * APPLY_MAXT];
+    __shared__ double s_ss[4 * 16 * T16], s_mx[4 * 16 * T16];
 #pragma unroll
-    for (int t = 0; t < APPLY_MAXT; ++t) {
+    for (int t = 0; t < T16; ++t) {
         double a = ss[t], b = mx[t];
         a += __shfl_xor(a, 16);
         b = fmax(b, __shfl_xor(b, 16));
         a += __shfl_xor(a, 32);
         b = fmax(b, __shfl_xor(b, 32));
         if (kq == 0) {
-            s_ss[(wave * APPLY_MAXT + t) * 16 + r16] = a;
-            s_mx[(wave * APPLY_MAXT + t) * 16 + r16] = b;
+            s_ss[(wave * T16 + t) * 16 + r16] = a;
+            s_mx[(wave * T16 + t) * 16 + r16] = b;
         }
     }
     __syncthreads();
@@ -446,11 +601,19 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
         const int t = c / 16, l = c % 16;
         double a = 0, b = -INFINITY;
         for (int w = 0; w < 4; ++w) {
-            a += s_ss[(w * APPLY_MAXT + t) * 16 + l];
-            b = fmax(b, s_mx[(w * APPLY_MAXT + t) * 16 + l]);
+            a += s_ss[(w * T16 + t) * 16 + l];
+            b = fmax(b, s_mx[(w * T16 + t) * 16 + l]);
         }
         psum[(size_t)blockIdx.x * R + c] = a;
         pmax[(size_t)blockIdx.x * R + c] = b;
+    }
+    if (pin) {
+        __shared__ double s_in[4];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) in += __shfl_xor(in, o);
+        if (lane == 0) s_in[wave] = in;
+        __syncthreads();
+        if (threadIdx.x == 0) pin[blockIdx.x] = (s_in[0] + s_in[1]) + (s_in[2] + s_in[3]);
     }
 }
 
@@ -516,15 +679,21 @@ __global__ void k_inner_partial(const UT* __restrict__ U, const MT* __restrict__
 
 // fit = 1 - sqrt(res)/sqrt(normX), res = normX + normU - 2 inner (sqrt only
 // when res > 0, as in the reference)
+// inner from per-column partials (nb x R, weighted by lambda) or, with
+// scalar set, from nb per-block totals already carrying lambda
 __global__ void k_fit(const double* __restrict__ grams, int nmode, int R, const double* __restrict__ lambda,
-                      const double* __restrict__ ipart, int nb, double normX, double* __restrict__ out)
+                      const double* __restrict__ ipart, int nb, int scalar, double normX, double* __restrict__ out)
 {
     __shared__ double s_in[256], s_nu[256];
     double inner = 0, nu = 0;
-    for (int c = threadIdx.x; c < R; c += blockDim.x) {
-        double a = 0;
-        for (int p = 0; p < nb; ++p) a += ipart[(size_t)p * R + c];
-        inner += a * lambda[c];
+    if (scalar) {
+        for (int p = threadIdx.x; p < nb; p += blockDim.x) inner += ipart[p];
+    } else {
+        for (int c = threadIdx.x; c < R; c += blockDim.x) {
+            double a = 0;
+            for (int p = 0; p < nb; ++p) a += ipart[(size_t)p * R + c];
+            inner += a * lambda[c];
+        }
     }
     for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
         const int i = k / R, j = k % R;
@@ -607,26 +776,44 @@ int chol_device(const double* grams, int nmode, int mode, int R, double* L, doub
         return ALTO_ERR_UNSUPPORTED;
     }
     (void)L;
-    hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(256), (size_t)R * R * 8, s, grams, nmode, mode, R, P, status);
-    hipLaunchKernelGGL(k_pinv, dim3(1), dim3(256), 0, s, grams, nmode, mode, R, wsA, wsQ, P, status);
+    // registers per thread: ceil(R / (BT / R)) matrix entries
+#define SPD_(BT, NPT) \
+    hipLaunchKernelGGL((k_spd_inverse<BT, NPT>), dim3(1), dim3(BT), 0, s, grams, nmode, mode, R, P, status, wsA, wsQ)
+    if (R <= 16) SPD_(256, 1);
+    else if (R <= 32) SPD_(1024, 1);
+    else if (R <= 64) SPD_(1024, 4);
+    else SPD_(1024, 16);
+#undef SPD_
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }
 
 template <typename MT, typename UT>
 int solve_device(const MT* M, uint64_t I, int R, const double* L, const double* P, const int* status, UT* U,
-                 double* psum, double* pmax, int nb, hipStream_t s)
+                 double* psum, double* pmax, int nb, hipStream_t s, double* pin)
 {
     (void)L;
     (void)status;
-    hipLaunchKernelGGL((k_apply<MT, UT>), dim3(nb), dim3(256), (size_t)R * (R + 2) * 8, s, M, I, R, P, U, psum, pmax);
+    const size_t lds = (size_t)R * (R + 2) * 8;
+#define AP_(TT) hipLaunchKernelGGL((k_apply<MT, UT, TT>), dim3(nb), dim3(256), lds, s, M, I, R, P, U, psum, pmax, pin)
+    switch ((R + 15) / 16) {
+    case 1: AP_(1); break;
+    case 2: AP_(2); break;
+    case 3: AP_(3); break;
+    case 4: AP_(4); break;
+    case 5: AP_(5); break;
+    case 6: AP_(6); break;
+    case 7: AP_(7); break;
+    default: AP_(8); break;
+    }
+#undef AP_
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }
 template int solve_device<double, double>(const double*, uint64_t, int, const double*, const double*, const int*,
-                                          double*, double*, double*, int, hipStream_t);
+                                          double*, double*, double*, int, hipStream_t, double*);
 template int solve_device<float, float>(const float*, uint64_t, int, const double*, const double*, const int*, float*,
-                                        double*, double*, int, hipStream_t);
+                                        double*, double*, int, hipStream_t, double*);
 
 int lambda_device(const double* psum, const double* pmax, int nb, int R, int which, double* lambda, hipStream_t s)
 {
@@ -665,7 +852,15 @@ template int inner_partial_device<float, float>(const float*, const float*, uint
 int fit_device(const double* grams, int nmode, int R, const double* lambda, const double* ipart, double normX,
                double* out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, grams, nmode, R, lambda, ipart, INNER_BLOCKS, normX, out);
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, grams, nmode, R, lambda, ipart, INNER_BLOCKS, 0, normX, out);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+int fit_scalar_device(const double* grams, int nmode, int R, const double* lambda, const double* pin, int nb,
+                      double normX, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fit, dim3(1), dim3(256), 0, s, grams, nmode, R, lambda, pin, nb, 1, normX, out);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }

@@ -31,8 +31,20 @@ static int g_seg = 0;
 int tuned_seg(uint64_t nnz, int G, int acc_bytes)
 {
     if (g_seg > 0) return g_seg;
+    if (nnz < (4ull << 20)) {
+        // below 4M nonzeros a launch is latency-bound: short segments give
+        // more lane groups.  Measured optimum on MI355X from 10K to 4M
+        // nonzeros (3-5 modes, ranks 8-64, FP64 and FP32): ~0.16 sqrt(nnz)
+        // nonzeros per group, as a power of two from 16
+        const double target = 0.16 * sqrt((double)nnz);
+        int seg = 16;
+        while (seg < 256 && seg * 1.5 < target) seg <<= 1;
+        return seg;
+    }
+    // large tensors: about 2^21 threads per launch (flushes cost more than
+    // the lost parallelism), twice as long segments with FP32 accumulation
     const uint64_t want = (nnz * (uint64_t)G >> 21) * (acc_bytes < 8 ? 2 : 1);
-    int seg = nnz < (4ull << 20) ? 256 : 512;  // small tensors keep more groups
+    int seg = 512;
     while (seg < 4096 && (uint64_t)seg * 3 / 2 < want) seg <<= 1;
     return seg;
 }
@@ -326,7 +338,7 @@ int alto_mttkrp_autotune(alto_tensor_t* t, uint64_t rank, int precision, const v
     hipStream_t s = (hipStream_t)stream;
     const int R = (int)rank;
     static const int variants[] = {2, 0, 1};
-    static const int segs[] = {256, 512, 1024, 2048, 4096};
+    static const int segs[] = {16, 32, 64, 128, 256, 512, 1024, 2048, 4096};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "alto_hip.h"
+#include "alto_internal.hpp"
 
 namespace {
 
@@ -117,6 +118,7 @@ struct alto_server {
                                     dev ? ALTO_PTR_DEVICE : ALTO_PTR_HOST, ALTO_VAL_F64, 0, 1, (alto_stream_t)s, &t);
         alto_cpd_session_t* sess = nullptr;
         if (st == ALTO_OK) st = alto_cpd_session_create_seeded(t, j->rank, j->precision, j->seed, (alto_stream_t)s, &sess);
+        if (st == ALTO_OK) st = alto::cpd_session_set_timing(sess, 0);  // phase times are not reported
         int ran = 0, conv = 0;
         if (st == ALTO_OK) st = alto_cpd_session_iterate(sess, j->max_iters, j->epsilon, nullptr, &ran, &conv, 0);
         if (st == ALTO_OK && j->want_factors) {

@@ -23,6 +23,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the serving leg runs 8 worker streams; HIP maps a process's streams onto
+# this many hardware queues (4 by default), so give each stream its own.
+# It must be set before the runtime initialises.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
 DIMS = [12092, 9184, 28818]
 NNZ = 76_900_000

@@ -64,10 +64,12 @@ def test_pseudo_inverse_cholesky(R):
     np.testing.assert_allclose(out, want, rtol=1e-8, atol=1e-10 * np.abs(want).max())
 
 
-def test_pseudo_inverse_fallback_matches_gelsy():
+@pytest.mark.parametrize("R", [8, 40, 100])
+def test_pseudo_inverse_fallback_matches_gelsy(R):
+    """The elimination fails on the zero pivot and the same launch computes
+    the minimum-norm pseudo-inverse (256- and 1024-thread launches)."""
     rng = np.random.default_rng(9)
-    R = 8
-    Us = [rng.random((d, R)) for d in (30, 40, 50)]
+    Us = [rng.random((d + R, R)) for d in (30, 40, 50)]
     Us[2][:, 5] = 0.0  # V has a zero row/column: Cholesky fails
     grams = [u.T @ u for u in Us]
     M = rng.random((40, R))

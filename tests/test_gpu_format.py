@@ -181,6 +181,25 @@ def test_mttkrp_tuning_knobs_keep_results(tune_reset, hot, variant, chunk_log2, 
             assert _rel(g.mttkrp(mode, U, A.PREC_BF16).astype(np.float64), mttkrp_numpy(cidx, vals, Ub, mode)) < 1e-4
 
 
+@pytest.mark.parametrize("seg", [1, 16, 37, 64, 256, 4096, 100000])
+@pytest.mark.parametrize("dims", [[300, 40, 700, 25], [500, 3000]], ids=lambda d: f"{len(d)}m")
+def test_mttkrp_segment_lengths_keep_results(tune_reset, seg, dims):
+    """Any segment length per lane group (one nonzero, not a batch multiple,
+    longer than the tensor) changes only the summation order."""
+    A = tune_reset
+    A.check(A.lib().alto_tune_set(1, seg), "tune_set")
+    cidx, vals = zipf_tensor(dims, 20000, 29)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    o = oracle.OracleAlto(dims, cidx, vals, W=1, nprtn=8)
+    for R in (8, 16, 64):
+        rng = np.random.default_rng(R)
+        U = [rng.random((d, R)) for d in dims]
+        for mode in range(len(dims)):
+            want = o.mttkrp(mode, U)
+            np.testing.assert_allclose(g.mttkrp(mode, U), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+            assert _rel(g.mttkrp(mode, U, A.PREC_F32).astype(np.float64), want) < 1e-4
+
+
 @pytest.mark.parametrize("prec", [0, 1, 2])
 def test_mttkrp_autotune_keeps_results(prec):
     """The autotuner times every lane mapping and segment length per mode,
@@ -196,7 +215,7 @@ def test_mttkrp_autotune_keeps_results(prec):
     res = g.mttkrp_autotune(R, prec, reps=2)
     assert [r["mode"] for r in res] == [0, 1, 2]
     for r in res:
-        assert r["variant"] in (-1, 0, 1, 2) and r["seg"] in (0, 256, 512, 1024, 2048, 4096)
+        assert r["variant"] in (-1, 0, 1, 2) and r["seg"] in (0, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096)
         assert r["ms"] <= r["default_ms"] + 1e-12
     rng = np.random.default_rng(4)
     U = [rng.random((d, R)) for d in dims]
