@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -60,6 +61,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise AltoError(f"{LIB_PATH} is missing: build it with `python -m alto_amd.build`")
+    # PyTorch's ROCm wheels carry a HIP runtime of their own, which finds no
+    # GPU when the other runtime in the process initialised first: when torch
+    # is loaded, let it start before this library makes its first HIP call
+    torch = sys.modules.get("torch")
+    if torch is not None:
+        try:
+            torch.cuda.is_available()
+        except Exception:
+            pass
     L = C.CDLL(LIB_PATH)
     vp, u64, i32, f64 = C.c_void_p, C.c_uint64, C.c_int, C.c_double
     sig = {

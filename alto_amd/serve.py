@@ -11,7 +11,7 @@ import time
 
 import numpy as np
 
-from . import PREC_F64, PTR_DEVICE, PTR_HOST, _ptr_array, check, lib
+from . import PREC_F64, PTR_DEVICE, PTR_HOST, AltoTensor, _ptr_array, check, lib
 
 
 class JobDesc(C.Structure):
@@ -167,3 +167,60 @@ def run_stream(jobs, precision, slots, budget=200 << 30, max_iters=20, epsilon=1
     return {"jobs": len(res), "jobs_per_s": len(res) / (last - first), "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "mean_iters": float(np.mean([r.iters for r in res])),
             "fits": [r.fit for r in res], "hbm_peak_gb": peak / 2 ** 30, "makespan_s": last - first}
+
+
+def run_sequential(jobs, precision, max_iters=20, epsilon=1e-5):
+    """The jobs one after another on the calling thread: create_alto and a
+    device-seeded CP-ALS session per job, as a server worker runs them, with
+    no admission or overlap (the path to profile kernel by kernel).  A job
+    holds either host arrays (cidx, vals) or device tensors (coords, vals)."""
+    L = lib()
+    fits, iters = [], 0
+    t0 = time.perf_counter()
+    for j in jobs:
+        if "cidx" in j:  # host COO (numpy)
+            g = AltoTensor(j["dims"], j["cidx"], j["vals"], nprtn=1)
+        else:  # device tensors from materialize()
+            g = AltoTensor.from_device(j["dims"], [c.data_ptr() for c in j["coords"]], j["vals"].data_ptr(), j["nnz"],
+                                       nprtn=1)
+        h = C.c_void_p()
+        check(L.alto_cpd_session_create_seeded(g.h, j["rank"], precision, j["seed"] % (2 ** 31), None, C.byref(h)),
+              "cpd_session_create_seeded")
+        fit = np.zeros(max_iters, np.float64)
+        ran, conv = C.c_int(0), C.c_int(0)
+        try:
+            check(L.alto_cpd_session_iterate(h, max_iters, epsilon, fit.ctypes.data, C.byref(ran), C.byref(conv), 0),
+                  "cpd_session_iterate")
+        finally:
+            L.alto_cpd_session_destroy(h)
+        fits.append(float(fit[ran.value - 1]))
+        iters += ran.value
+        del g
+    dt = time.perf_counter() - t0
+    return {"jobs": len(jobs), "jobs_per_s": len(jobs) / dt, "mean_iters": iters / len(jobs), "fits": fits,
+            "makespan_s": dt}
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="CP-ALS job stream through the server or one job at a time")
+    ap.add_argument("--jobs", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--precision", type=int, default=PREC_F64, help="0 FP64, 1 FP32, 2 BF16")
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--sequential", action="store_true", help="no server: jobs one after another")
+    ap.add_argument("--time-scale", type=float, default=0.0, help="arrival-time scale (0: all jobs at once)")
+    args = ap.parse_args(argv)
+    jobs = materialize(job_stream(args.jobs, seed=args.seed))
+    if args.sequential:
+        r = run_sequential(jobs, args.precision)
+    else:
+        r = run_stream(jobs, args.precision, args.slots, time_scale=args.time_scale)
+    r.pop("fits", None)
+    print(json.dumps({"precision": args.precision, "sequential": args.sequential, "slots": args.slots, **r}),
+          flush=True)
+
+
+if __name__ == "__main__":
+    main()

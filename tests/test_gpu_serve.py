@@ -64,3 +64,22 @@ def test_budget_admission_and_error_isolation():
     st = s.stats()
     assert st.hbm_peak <= max(st.hbm_budget, max(s.result(i)[0].footprint for i in ids))
     s.close()
+
+
+def test_sequential_runner_equals_server():
+    """The one-job-at-a-time runner (device-seeded sessions on the calling
+    thread) and the server with 4 slots give every job of a seeded stream
+    the same fit (FP64, within 1e-10 relative)."""
+    from alto_amd.serve import Server, run_sequential
+    jobs = _jobs(10, seed=2)
+    seq = run_sequential(jobs, 0, max_iters=15, epsilon=1e-5)
+    s4 = Server(8 << 30, 4)
+    ids = [s4.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], 15, 1e-5, j["seed"], 0) for j in jobs]
+    fits = []
+    for jid in ids:
+        r = s4.result(jid)[0]
+        assert r.status == 2
+        fits.append(r.fit)
+    s4.close()
+    assert seq["jobs"] == 10
+    np.testing.assert_allclose(seq["fits"], fits, rtol=1e-10, atol=1e-12)
