@@ -184,7 +184,8 @@ static int cpd_mode_step(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it,
     const void* fptr[ALTO_MAXN];
     for (int m = 0; m < N; ++m) fptr[m] = c.st(m);
     ALTO_TRY(pt.mark(-1));
-    ALTO_TRY(mttkrp_device(t, j, R, fptr, c.M.p, c.prec, 0, s, 0));
+    // M is zero here: cleared at set-up and by the previous mode's last launch
+    ALTO_TRY(mttkrp_device(t, j, R, fptr, c.M.p, c.prec, 1, s, 0));
     ALTO_TRY(pt.mark(PH_MTTKRP));
     ALTO_TRY(chol_device(c.grams.template as<double>(), N, j, R, c.L.template as<double>(), c.P.template as<double>(),
                          c.wsA.template as<double>(), c.wsQ.template as<double>(), c.status.template as<int>(), s));
@@ -197,10 +198,13 @@ static int cpd_mode_step(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it,
     ALTO_TRY(lambda_device(c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, R, it == 0 ? 0 : 1,
                            c.lambda.template as<double>(), s));
     ALTO_TRY(pt.mark(PH_NORM));
+    // ... and clears M for the next mode's MTTKRP in the same launches
+    const uint64_t next_rows = t->dims[(j + 1) % N];
     ALTO_TRY((gram_scale_device<UT, ST>(c.u(j), t->dims[j], R, c.lambda.template as<double>(),
                                         std::is_same<UT, ST>::value ? nullptr : c.st(j),
                                         c.grams.template as<double>() + (size_t)j * R * R,
-                                        c.gpart.template as<double>(), s, gram_f64(c.prec))));
+                                        c.gpart.template as<double>(), s, gram_f64(c.prec), c.M.p,
+                                        next_rows * R * sizeof(MT))));
     ALTO_TRY(pt.mark(PH_UPDATE));
     return ALTO_OK;
 }
@@ -301,6 +305,7 @@ struct CpdSession : CpdSessionBase {
         ALTO_TRY(cpd_alloc(t, R, c));
         ALTO_TRY(cpd_upload(t, c, Uh, s, dev_seed));
         ALTO_TRY(fill_value<double>(c.lambda.template as<double>(), R, 1.0, s));
+        ALTO_TRY(fill_value<MT>(c.M.template as<MT>(), t->dims[0] * R, (MT)0, s));  // the first MTTKRP accumulates
         ALTO_TRY(normsq_device(t->vals.as<double>(), t->nnz, c.nxpart.template as<double>(), c.nx.template as<double>(), s));
         HIP_TRY(hipMemcpyAsync(&normX, c.nx.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));

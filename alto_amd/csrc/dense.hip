@@ -204,10 +204,23 @@ __global__ void k_sum_partials(const double* __restrict__ part, int nparts, int 
 
 // out[k] = sum_p part[p n + k]: a block covers 32 consecutive k, its 8
 // wave halves stride over the partials, and the 8 sums are added in a fixed
-// order (deterministic)
+// order (deterministic).  Blocks from nsum on clear `zero` (nz bytes, a
+// multiple of 4): a buffer the caller needs zeroed next, for free in this
+// launch.
 __global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ part, int nparts, int n,
-                                                  double* __restrict__ out)
+                                                  double* __restrict__ out, int nsum, void* __restrict__ zero,
+                                                  uint64_t nz)
 {
+    if ((int)blockIdx.x >= nsum) {
+        const uint64_t t = (uint64_t)(blockIdx.x - nsum) * blockDim.x + threadIdx.x;
+        const uint64_t stride = (uint64_t)(gridDim.x - nsum) * blockDim.x;
+        uint4* z16 = reinterpret_cast<uint4*>(zero);
+        const uint64_t n16 = nz / 16;
+        for (uint64_t i = t; i < n16; i += stride) z16[i] = make_uint4(0, 0, 0, 0);
+        uint32_t* z4 = reinterpret_cast<uint32_t*>(zero);
+        for (uint64_t i = n16 * 4 + t; i < nz / 4; i += stride) z4[i] = 0;
+        return;
+    }
     __shared__ double s[8][33];
     const int kk = threadIdx.x & 31, pg = threadIdx.x >> 5;
     const int k = blockIdx.x * 32 + kk;
@@ -222,6 +235,22 @@ __global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ par
         for (int g = 0; g < 8; ++g) v += s[g][kk];
         out[k] = v;
     }
+}
+
+// the column sums of n-entry partials (n > 0) and the clearing of `zero`
+// (zero_bytes, a multiple of 4) in one launch
+int sum_cols_device(const double* part, int nparts, int n, double* out, void* zero, uint64_t zero_bytes,
+                    hipStream_t s)
+{
+    const int nsum = (n + 31) / 32;
+    uint64_t nzb = (zero_bytes / 16 + 255) / 256;
+    if (nzb > 1024) nzb = 1024;
+    if (zero_bytes && nzb == 0) nzb = 1;
+    if (nsum + nzb == 0) return ALTO_OK;
+    hipLaunchKernelGGL(k_sum_cols, dim3((unsigned)(nsum + nzb)), dim3(256), 0, s, part, nparts, n, out, nsum, zero,
+                       zero_bytes);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
 }
 
 int gram_blocks(uint64_t I)
@@ -264,18 +293,19 @@ int gram_device(const T* U, uint64_t I, int R, double* G, double* part, hipStrea
     } else {
         hipLaunchKernelGGL((k_gram_partial<T, false>), dim3(nb), dim3(256), 0, s, U, I, R, part);
     }
-    hipLaunchKernelGGL(k_sum_cols, dim3((R * R + 31) / 32), dim3(256), 0, s, part, nb, R * R, G);
-    HIP_TRY(hipGetLastError());
+    ALTO_TRY(sum_cols_device(part, nb, R * R, G, nullptr, 0, s));
     return ALTO_OK;
 }
 template <typename UT, typename ST>
 int gram_scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, double* G, double* part, hipStream_t s,
-                      bool f64)
+                      bool f64, void* zero, uint64_t zero_bytes)
 {
     constexpr bool same = std::is_same<UT, ST>::value;
     if (R > 64) {
         ALTO_TRY((scale_device<UT, ST>(U, I, R, lambda, same ? nullptr : S, s)));
-        return gram_device<ST>(same ? reinterpret_cast<const ST*>(U) : S, I, R, G, part, s, f64);
+        ALTO_TRY(gram_device<ST>(same ? reinterpret_cast<const ST*>(U) : S, I, R, G, part, s, f64));
+        if (zero_bytes) ALTO_TRY(sum_cols_device(nullptr, 0, 0, nullptr, zero, zero_bytes, s));
+        return ALTO_OK;
     }
     const int nb = gram_blocks(I);
     const uint64_t rpb = ((I + nb - 1) / nb + 15) / 16 * 16;
@@ -296,16 +326,14 @@ int gram_scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, dou
     default: GS_(4); break;
     }
 #undef GS_
-    hipLaunchKernelGGL(k_sum_cols, dim3((R * R + 31) / 32), dim3(256), 0, s, part, nb, R * R, G);
-    HIP_TRY(hipGetLastError());
-    return ALTO_OK;
+    return sum_cols_device(part, nb, R * R, G, zero, zero_bytes, s);
 }
 template int gram_scale_device<double, double>(double*, uint64_t, int, const double*, double*, double*, double*,
-                                               hipStream_t, bool);
+                                               hipStream_t, bool, void*, uint64_t);
 template int gram_scale_device<float, float>(float*, uint64_t, int, const double*, float*, double*, double*,
-                                             hipStream_t, bool);
+                                             hipStream_t, bool, void*, uint64_t);
 template int gram_scale_device<float, bf16_t>(float*, uint64_t, int, const double*, bf16_t*, double*, double*,
-                                              hipStream_t, bool);
+                                              hipStream_t, bool, void*, uint64_t);
 
 template int gram_device<double>(const double*, uint64_t, int, double*, double*, hipStream_t, bool);
 template int gram_device<float>(const float*, uint64_t, int, double*, double*, hipStream_t, bool);
@@ -322,11 +350,12 @@ template int gram_device<bf16_t>(const bf16_t*, uint64_t, int, double*, double*,
 // Thread t keeps the entries (r0 + rstep n, jc) of the matrix in registers,
 // rstep = BT / R, r0 = t / R, jc = t % R (BT threads: 1024 from R = 17 on,
 // so that a thread holds at most 16 entries and the waves of all four SIMDs
-// share each step).  Step k publishes column k (the
-// elimination factors, pivot included) and row k through LDS and updates
-// every register; the two LDS buffers alternate between steps, so one
-// barrier per step orders them (a thread writes buffer k & 1 at step k only
-// after the barrier of step k - 1, which every reader of step k - 2 passed).
+// share each step).  Step k reads column k (the elimination factors, pivot
+// included) and row k from LDS buffer k & 1 and updates every register; the
+// owners of column and row k + 1 write their new values into the other
+// buffer as they compute them, so one barrier per step orders everything
+// (buffer (k + 1) & 1 is written after the barrier of step k, which every
+// reader of step k - 1, the buffer's previous use, has passed).
 __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ grams, int nmode, int mode, int R,
                                                     double* __restrict__ A, double* __restrict__ Q,
                                                     double* __restrict__ Pout);
@@ -354,23 +383,19 @@ __global__ void __launch_bounds__(BT) k_spd_inverse(const double* __restrict__ g
             for (int n = 0; n < NPT; ++n) a[n] *= g[rc(n) * R];
         }
     }
-    // the step at which this thread publishes row k: k = r0 + rstep * n
+    // column 0 and row 0 for the first step; every later step's column and
+    // row are written by their owners as they update them
+    if (active) {
+#pragma unroll
+        for (int n = 0; n < NPT; ++n) {
+            const int i = r0 + rstep * n;
+            if (i < R && jc == 0) s_col[0][i] = a[n];
+        }
+        if (r0 == 0) s_row[0][jc] = a[0];
+    }
     int ok = 1;
     for (int k = 0; k < R; ++k) {
-        const int b = k & 1;
-        if (active) {
-            if (jc == k) {
-#pragma unroll
-                for (int n = 0; n < NPT; ++n)
-                    if (r0 + rstep * n < R) s_col[b][r0 + rstep * n] = a[n];
-            }
-            if (k >= r0 && (k - r0) % rstep == 0) {
-                double rv = 0.0;
-#pragma unroll
-                for (int n = 0; n < NPT; ++n) rv = r0 + rstep * n == k ? a[n] : rv;
-                s_row[b][jc] = rv;
-            }
-        }
+        const int b = k & 1, nb = b ^ 1;
         __syncthreads();
         const double piv = s_col[b][k];
         if (!(piv > 0.0)) {  // uniform: every thread read the same pivot
@@ -387,8 +412,15 @@ __global__ void __launch_bounds__(BT) k_spd_inverse(const double* __restrict__ g
 #pragma unroll
             for (int n = 0; n < CH; ++n) cv[n] = s_col[b][rc(n0 + n)];
 #pragma unroll
-            for (int n = 0; n < CH; ++n)
-                a[n0 + n] = r0 + rstep * (n0 + n) == k ? rk : (jc == k ? 0.0 : a[n0 + n]) - cv[n] * rk;
+            for (int n = 0; n < CH; ++n) {
+                const int i = r0 + rstep * (n0 + n);
+                const double v = i == k ? rk : (jc == k ? 0.0 : a[n0 + n]) - cv[n] * rk;
+                a[n0 + n] = v;
+                if (active && i < R) {
+                    if (jc == k + 1) s_col[nb][i] = v;
+                    if (i == k + 1) s_row[nb][jc] = v;
+                }
+            }
         }
     }
     if (ok) {

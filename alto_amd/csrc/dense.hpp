@@ -19,10 +19,14 @@ int lambda_device(const double* psum, const double* pmax, int nb, int R, int whi
 template <typename UT, typename ST>
 int scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, hipStream_t s);
 // U /= lambda (per column) with the storage copy S (null when the types
-// agree), then the Gram of the stored values, in one pass for R <= 64
+// agree), then the Gram of the stored values, in one pass for R <= 64;
+// zero_bytes of `zero` are cleared in the same launches
 template <typename UT, typename ST>
 int gram_scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, double* G, double* part, hipStream_t s,
-                      bool f64);
+                      bool f64, void* zero = nullptr, uint64_t zero_bytes = 0);
+// column sums of partials and the clearing of a buffer, one launch
+int sum_cols_device(const double* part, int nparts, int n, double* out, void* zero, uint64_t zero_bytes,
+                    hipStream_t s);
 template <typename UT, typename MT>
 int inner_partial_device(const UT* U, const MT* Mm, uint64_t I, int R, double* part, hipStream_t s);
 int fit_device(const double* grams, int nmode, int R, const double* lambda, const double* ipart, double normX,
