@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(BT) k_spd_inverse(const double* __restrict__ g
 }
 
 // Minimum-norm pseudo-inverse V^+ = Q diag(1/e) Q^T over |e| > R*eps*max|e|
-// (cyclic Jacobi; runs only after a failed Cholesky).  A and Q live in the
+// (parallel cyclic Jacobi; runs only after a failed Cholesky).  A and Q live in the
 // workspace (global memory), shared through work-group barriers.
 __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ grams, int nmode, int mode, int R,
                                                     double* __restrict__ A, double* __restrict__ Q,
@@ -446,7 +446,8 @@ __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ g
 {
     const int tid = threadIdx.x;
     __shared__ double s_red[1024];
-    __shared__ double cs[2];
+    __shared__ double s_c[64], s_s[64];
+    __shared__ int s_p[64], s_q[64];
     __shared__ int s_stop;
     for (int k = tid; k < R * R; k += blockDim.x) {
         const int i = k / R, c = k % R;
@@ -493,40 +494,59 @@ __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ g
         if (tid == 0) s_stop = (offs <= 1e-30 * s_red[0]) || offs == 0.0;
         __syncthreads();
         if (s_stop) break;
-        for (int p = 0; p < R - 1; ++p)
-            for (int q = p + 1; q < R; ++q) {
-                if (tid == 0) {
+        // one sweep in Rp - 1 steps of Rp / 2 disjoint rotations (round-robin
+        // pairing: index Rp - 1 stays, the others circulate, so every pair
+        // meets once per sweep; with R odd, index R is a dummy).  A step
+        // computes all its rotations from the same A, then applies
+        // A <- A J, Q <- Q J and A <- J^T A, each in parallel.
+        const int Rp = R + (R & 1);
+        for (int t = 0; t < Rp - 1; ++t) {
+            for (int k = tid; k < Rp / 2; k += blockDim.x) {
+                const int u = k == 0 ? Rp - 1 : (t + k) % (Rp - 1);
+                const int v = (t - k + Rp - 1) % (Rp - 1);
+                const int p = min(u, v), q = max(u, v);
+                double c = 1.0, sn = 0.0;
+                if (q < R) {
                     const double apq = A[p * R + q];
-                    if (fabs(apq) < 1e-300) {
-                        cs[0] = 1.0;
-                        cs[1] = 0.0;
-                    } else {
+                    if (fabs(apq) >= 1e-300) {
                         const double th = (A[q * R + q] - A[p * R + p]) / (2.0 * apq);
-                        const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                        cs[0] = 1.0 / sqrt(t * t + 1.0);
-                        cs[1] = t * cs[0];
+                        const double tt = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                        c = 1.0 / sqrt(tt * tt + 1.0);
+                        sn = tt * c;
                     }
                 }
-                __syncthreads();
-                const double c = cs[0], s = cs[1];
-                if (s != 0.0) {
-                    for (int k = tid; k < R; k += blockDim.x) {
-                        const double akp = A[k * R + p], akq = A[k * R + q];
-                        A[k * R + p] = c * akp - s * akq;
-                        A[k * R + q] = s * akp + c * akq;
-                    }
-                    __syncthreads();
-                    for (int k = tid; k < R; k += blockDim.x) {
-                        const double apk = A[p * R + k], aqk = A[q * R + k];
-                        A[p * R + k] = c * apk - s * aqk;
-                        A[q * R + k] = s * apk + c * aqk;
-                        const double qkp = Q[k * R + p], qkq = Q[k * R + q];
-                        Q[k * R + p] = c * qkp - s * qkq;
-                        Q[k * R + q] = s * qkp + c * qkq;
-                    }
-                }
-                __syncthreads();
+                s_p[k] = p;
+                s_q[k] = q;
+                s_c[k] = c;
+                s_s[k] = sn;
             }
+            __syncthreads();
+            for (int w = tid; w < (Rp / 2) * R; w += blockDim.x) {  // columns p, q of A and Q
+                const int k = w / R, r = w % R;
+                const double c = s_c[k], sn = s_s[k];
+                if (sn != 0.0) {
+                    const int p = s_p[k], q = s_q[k];
+                    const double arp = A[r * R + p], arq = A[r * R + q];
+                    A[r * R + p] = c * arp - sn * arq;
+                    A[r * R + q] = sn * arp + c * arq;
+                    const double qrp = Q[r * R + p], qrq = Q[r * R + q];
+                    Q[r * R + p] = c * qrp - sn * qrq;
+                    Q[r * R + q] = sn * qrp + c * qrq;
+                }
+            }
+            __syncthreads();
+            for (int w = tid; w < (Rp / 2) * R; w += blockDim.x) {  // rows p, q of A
+                const int k = w / R, r = w % R;
+                const double c = s_c[k], sn = s_s[k];
+                if (sn != 0.0) {
+                    const int p = s_p[k], q = s_q[k];
+                    const double apr = A[p * R + r], aqr = A[q * R + r];
+                    A[p * R + r] = c * apr - sn * aqr;
+                    A[q * R + r] = sn * apr + c * aqr;
+                }
+            }
+            __syncthreads();
+        }
     }
     double emax = 0;
     for (int k = 0; k < R; ++k) emax = fmax(emax, fabs(A[k * R + k]));
