@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "alto/alto.hpp"
+#include "tensor_io.hpp"
 
 static const char* short_opt = "hvi:o:b:l:r:m:n:x:d:t:s:e:f:zcp";
 static const struct option long_opt[] = {
@@ -54,108 +55,6 @@ static void usage(const char* call)
     fprintf(stderr, "\t-s or --sparsity     Sparsity of generate tensor\n");
     fprintf(stderr, "\t-c or --check        Run ALTO (par) validation against cpd MTTKRP\n");
     fprintf(stderr, "\t-p or --bench        Run ALTO benchmark mode based on the CPD model with the given CMD line options\n");
-}
-
-// .tns text reader (src/sptensor.cpp:75-161): dims are the largest index
-// (+ zero_based), '#' lines and empty lines are skipped, values via strtod.
-static SparseTensor* read_tns(const char* path, int zero_based)
-{
-    FILE* fp = fopen(path, "r");
-    if (!fp) {
-        fprintf(stderr, "cannot open %s\n", path);
-        exit(1);
-    }
-    char* line = NULL;
-    size_t len = 0;
-    ssize_t nread;
-    int nmodes = -1;
-    std::vector<std::vector<IType>> idx;
-    std::vector<FType> vals;
-    while ((nread = getline(&line, &len, fp)) != -1) {
-        if (nread <= 1 || line[0] == '#') continue;
-        if (nmodes < 0) {
-            int tok = 0;
-            char* copy = strdup(line);
-            for (char* p = strtok(copy, " \t\n"); p; p = strtok(NULL, " \t\n")) ++tok;
-            free(copy);
-            nmodes = tok - 1;
-            if (nmodes < 1 || nmodes > MAX_NUM_MODES) {
-                fprintf(stderr, "unsupported number of modes %d\n", nmodes);
-                exit(1);
-            }
-            idx.resize(nmodes);
-        }
-        char* p = line;
-        for (int m = 0; m < nmodes; ++m) idx[m].push_back(strtoull(p, &p, 10) - (zero_based ? 0 : 1));
-        vals.push_back(strtod(p, &p));
-    }
-    free(line);
-    fclose(fp);
-    SparseTensor* X = new SparseTensor();
-    X->nmodes = nmodes;
-    X->nnz = vals.size();
-    X->dims = new IType[nmodes];
-    X->cidx = new IType*[nmodes];
-    for (int m = 0; m < nmodes; ++m) {
-        IType mx = 0;
-        X->cidx[m] = new IType[X->nnz];
-        for (IType i = 0; i < X->nnz; ++i) {
-            X->cidx[m][i] = idx[m][i];
-            if (idx[m][i] + 1 > mx) mx = idx[m][i] + 1;
-        }
-        X->dims[m] = mx;
-    }
-    X->vals = new FType[X->nnz];
-    memcpy(X->vals, vals.data(), X->nnz * sizeof(FType));
-    return X;
-}
-
-// binary reader (src/sptensor.cpp:211-258): nmodes (8 bytes, low 32 bits
-// meaningful), dims, nnz, the index arrays, the values
-static SparseTensor* read_bin(const char* path)
-{
-    FILE* fp = fopen(path, "rb");
-    if (!fp) {
-        fprintf(stderr, "cannot open %s\n", path);
-        exit(1);
-    }
-    IType nm = 0, nnz = 0;
-    if (fread(&nm, 8, 1, fp) != 1) exit(1);
-    nm &= 0xffffffffull;
-    SparseTensor* X = new SparseTensor();
-    X->nmodes = (int)nm;
-    X->dims = new IType[nm];
-    if (fread(X->dims, 8, nm, fp) != nm || fread(&nnz, 8, 1, fp) != 1) exit(1);
-    X->nnz = nnz;
-    X->cidx = new IType*[nm];
-    for (IType m = 0; m < nm; ++m) {
-        X->cidx[m] = new IType[nnz];
-        if (fread(X->cidx[m], 8, nnz, fp) != nnz) exit(1);
-    }
-    X->vals = new FType[nnz];
-    if (fread(X->vals, 8, nnz, fp) != nnz) exit(1);
-    fclose(fp);
-    return X;
-}
-
-static void write_tensor(const char* path, SparseTensor* X, bool binary)
-{
-    FILE* fp = fopen(path, binary ? "wb" : "w");
-    if (!fp) return;
-    if (binary) {
-        IType nm = (IType)X->nmodes;
-        fwrite(&nm, 8, 1, fp);
-        fwrite(X->dims, 8, X->nmodes, fp);
-        fwrite(&X->nnz, 8, 1, fp);
-        for (int m = 0; m < X->nmodes; ++m) fwrite(X->cidx[m], 8, X->nnz, fp);
-        fwrite(X->vals, 8, X->nnz, fp);
-    } else {
-        for (IType i = 0; i < X->nnz; ++i) {
-            for (int m = 0; m < X->nmodes; ++m) fprintf(fp, "%llu ", X->cidx[m][i] + 1);
-            fprintf(fp, "%g\n", X->vals[i]);
-        }
-    }
-    fclose(fp);
 }
 
 static void print_info(IType rank, int max_iters, SparseTensor* X)
@@ -269,7 +168,7 @@ int main(int argc, char** argv)
         case 'h': usage(argv[0]); return 0;
         case 'v': fprintf(stdout, "%s version 0.1.2 (%s)\n", argv[0], alto_version()); return 0;
         case 'i': text_file = optarg; break;
-        case 'o': text_out = optarg; break;
+        case 'o': text_out = optarg; break;  // accepted and unused, as in the reference
         case 'b': bin_file = optarg; break;
         case 'l':
             if (!strcmp(optarg, "ALS") || !strcmp(optarg, "als")) model = ALS;
@@ -291,13 +190,14 @@ int main(int argc, char** argv)
         default: usage(argv[0]); return -1;
         }
     }
+    (void)text_out;
     SparseTensor* X = nullptr;
     if (!bin_file.empty()) {
         X = read_bin(bin_file.c_str());
-        if (save) write_tensor(text_out.empty() ? "out.tns" : text_out.c_str(), X, false);
+        if (save) write_tensor(export_name(X, false).c_str(), X, false);
     } else if (!text_file.empty()) {
         X = read_tns(text_file.c_str(), zero_based);
-        if (save) write_tensor(text_out.empty() ? "out.bin" : text_out.c_str(), X, true);
+        if (save) write_tensor(export_name(X, true).c_str(), X, true);
     } else {
         fprintf(stderr, dims_arg.empty() ? "No dims specified... exiting\n"
                                          : "-d/-s tensor generation is not part of this build; pass -i or -b\n");

@@ -62,3 +62,97 @@ def test_cli_check_bench_and_run(tmp_path):
     r = subprocess.run([os.path.join(BIN, "cpd64"), "-l", "APR", "-i", str(tns), "-r", "4", "-m", "3", "-n", "2",
                         "-x", "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "Final log-likelihood" in r.stdout, r.stdout + r.stderr
+    # -f: the text input is also written as <N>D_<nnz>.bin; running from that
+    # binary file gives the same fit trace
+    runs = []
+    for args in (["-i", str(tns), "-f", "1"], ["-b", "3D_4000.bin"]):
+        r = subprocess.run([os.path.join(BIN, "cpd64")] + args + ["-r", "8", "-m", "4", "-x", "1"], cwd=tmp_path,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        runs.append([l.split("\t")[1] for l in r.stdout.splitlines() if l.startswith("it: ")])
+    assert (tmp_path / "3D_4000.bin").exists()
+    assert len(runs[0]) == 4 and runs[0] == runs[1]
+
+
+_IO_PROG = r'''
+#include "tensor_io.hpp"
+// t/z: read .tns (one-/zero-based); b: read binary; w: .tns -> binary under
+// the export name; x: binary -> .tns under the export name.  Prints the
+// tensor (nmodes nnz dims, then one line per nonzero) or the written name.
+int main(int argc, char** argv) {
+    const char k = argv[1][0];
+    SparseTensor* X = (k == 'b' || k == 'x') ? read_bin(argv[2]) : read_tns(argv[2], k == 'z');
+    if (k == 'w' || k == 'x') {
+        const std::string fn = export_name(X, k == 'w');
+        write_tensor(fn.c_str(), X, k == 'w');
+        printf("%s\n", fn.c_str());
+        return 0;
+    }
+    printf("%d %llu", X->nmodes, (unsigned long long)X->nnz);
+    for (int m = 0; m < X->nmodes; ++m) printf(" %llu", (unsigned long long)X->dims[m]);
+    printf("\n");
+    for (IType i = 0; i < X->nnz; ++i) {
+        for (int m = 0; m < X->nmodes; ++m) printf("%llu ", (unsigned long long)X->cidx[m][i]);
+        printf("%.17g\n", X->vals[i]);
+    }
+    return 0;
+}
+'''
+
+
+def _io_tool(tmp_path):
+    src = tmp_path / "io.cpp"
+    src.write_text(_IO_PROG)
+    exe = tmp_path / "io"
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(ROOT, "alto_amd", "cli"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return exe
+
+
+def _io_read(exe, kind, path, cwd):
+    r = subprocess.run([str(exe), kind, str(path)], capture_output=True, text=True, cwd=cwd)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    head = [int(x) for x in lines[0].split()]
+    nm, nnz, dims = head[0], head[1], head[2:]
+    rows = [l.split() for l in lines[1:]]
+    cidx = np.array([[int(r[m]) for r in rows] for m in range(nm)], dtype=np.uint64)
+    vals = np.array([float(r[nm]) for r in rows])
+    return nm, nnz, dims, cidx, vals
+
+
+def test_tensor_io_formats(tmp_path):
+    """The command line's tensor files follow the reference (src/sptensor.cpp):
+    .tns dims are the largest index (one-based) or largest + 1 (-z), '#' and
+    empty lines are skipped; the binary file round-trips exactly, its 8-byte
+    mode count is read through its low 32 bits, and exports are named
+    <N>D_<nnz>.bin / .tns with "%g" text values."""
+    import struct
+    exe = _io_tool(tmp_path)
+    tns = tmp_path / "a.tns"
+    tns.write_text("# comment\n\n1 2 3 1.5\n4 1 2 0.125\n\n# another\n2 7 1 3.0000000000000004\n")
+    nm, nnz, dims, cidx, vals = _io_read(exe, "t", tns, tmp_path)
+    assert (nm, nnz, dims) == (3, 3, [4, 7, 3])
+    np.testing.assert_array_equal(cidx, [[0, 3, 1], [1, 0, 6], [2, 1, 0]])
+    np.testing.assert_array_equal(vals, [1.5, 0.125, 3.0000000000000004])
+    _, _, dims_z, cidx_z, _ = _io_read(exe, "z", tns, tmp_path)
+    assert dims_z == [5, 8, 4]
+    np.testing.assert_array_equal(cidx_z, cidx + 1)
+    # .tns -> binary under the reference's export name, read back exactly
+    r = subprocess.run([str(exe), "w", str(tns)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0 and r.stdout.strip() == "3D_3.bin", r.stdout + r.stderr
+    _, _, dims_b, cidx_b, vals_b = _io_read(exe, "b", tmp_path / "3D_3.bin", tmp_path)
+    assert dims_b == dims and np.array_equal(cidx_b, cidx) and np.array_equal(vals_b, vals)
+    # a binary file whose mode count carries garbage in its upper 4 bytes
+    d = np.array([4, 7, 3], np.uint64)
+    blob = struct.pack("<Q", (0xDEADBEEF << 32) | 3) + d.tobytes() + struct.pack("<Q", 3)
+    blob += cidx.astype(np.uint64).tobytes() + vals.tobytes()
+    (tmp_path / "g.bin").write_bytes(blob)
+    nm, nnz, dims_g, cidx_g, vals_g = _io_read(exe, "b", tmp_path / "g.bin", tmp_path)
+    assert (nm, nnz, dims_g) == (3, 3, [4, 7, 3]) and np.array_equal(cidx_g, cidx) and np.array_equal(vals_g, vals)
+    # binary -> .tns: one-based indices, "%g" values
+    r = subprocess.run([str(exe), "x", str(tmp_path / "g.bin")], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 0 and r.stdout.strip() == "3D_3.tns"
+    assert (tmp_path / "3D_3.tns").read_text().splitlines() == ["1 2 3 1.5", "4 1 2 0.125", "2 7 1 3"]
