@@ -15,7 +15,10 @@
 #include <string>
 #include <vector>
 
+#include <thread>
+
 #include "alto/alto.hpp"
+#include "poisson_gen.hpp"
 #include "tensor_io.hpp"
 
 static const char* short_opt = "hvi:o:b:l:r:m:n:x:d:t:s:e:f:zcp";
@@ -157,7 +160,7 @@ int main(int argc, char** argv)
 {
     int max_iters = 10, max_inner = 10, target_mode = -1, save = 0, zero_based = 0;
     IType rank = 16;
-    double epsilon = 1e-5;
+    double epsilon = 1e-5, sparsity = 0.1;
     int seed = (int)time(NULL);
     bool do_check = false, do_bench = false;
     Model model = ALS;
@@ -181,7 +184,10 @@ int main(int argc, char** argv)
         case 'x': seed = atoi(optarg); if (seed < 0) { fprintf(stderr, "Invalid -seed: %s.\n", optarg); return -1; } break;
         case 'd': dims_arg = optarg; break;
         case 't': target_mode = atoi(optarg); if (target_mode < -1) { fprintf(stderr, "Invalid -target-mode: %s.\n", optarg); return -1; } break;
-        case 's': break;
+        case 's':
+            sparsity = atof(optarg);
+            if (sparsity <= 0.0) { fprintf(stderr, "Invalid -sparsity: %s.\n", optarg); return -1; }
+            break;
         case 'e': epsilon = atof(optarg); if (epsilon <= 0.0) { fprintf(stderr, "Invalid -epsilon: %s.\n", optarg); return -1; } break;
         case 'f': save = atoi(optarg); if (save < 0) { fprintf(stderr, "Invalid -file: %s.\n", optarg); return -1; } break;
         case 'z': zero_based = 1; break;
@@ -198,11 +204,48 @@ int main(int argc, char** argv)
     } else if (!text_file.empty()) {
         X = read_tns(text_file.c_str(), zero_based);
         if (save) write_tensor(export_name(X, true).c_str(), X, true);
-    } else {
-        fprintf(stderr, dims_arg.empty() ? "No dims specified... exiting\n"
-                                         : "-d/-s tensor generation is not part of this build; pass -i or -b\n");
+    } else if (dims_arg.empty()) {
+        fprintf(stderr, "No dims specified... exiting\n");
         usage(argv[0]);
         return -1;
+    } else {
+        // MakeSparseTensor (src/main.cpp:653-675): sparsity x prod(dims) Poisson
+        // draws from a random rank-`rank` model; the reference splits its
+        // uniform draws over its OpenMP threads, so the thread count
+        // (OMP_NUM_THREADS, else the hardware threads) selects the tensor
+        std::vector<IType> dims;
+        size_t pos = 0;
+        while (pos != std::string::npos) {
+            const size_t e = dims_arg.find(',', pos);
+            dims.push_back((IType)atoll(dims_arg.substr(pos, e == std::string::npos ? e : e - pos).c_str()));
+            pos = e == std::string::npos ? e : e + 1;
+        }
+        IType cells = 1;
+        for (IType d : dims) {
+            if (d < rank) {
+                fprintf(stderr, "Invalid -dims: every dimension must be at least the rank (%llu)\n",
+                        (unsigned long long)rank);
+                return -1;
+            }
+            cells *= d;
+        }
+        if (dims.empty() || dims.size() > MAX_NUM_MODES) {
+            fprintf(stderr, "Invalid -dims: %s.\n", dims_arg.c_str());
+            return -1;
+        }
+        const IType edges = (IType)(sparsity * (double)cells);
+        if (edges < 1) {
+            fprintf(stderr, "Invalid -sparsity: no nonzeros for %s.\n", dims_arg.c_str());
+            return -1;
+        }
+        int threads = (int)std::thread::hardware_concurrency();
+        if (const char* e = getenv("OMP_NUM_THREADS")) threads = atoi(e);
+        if (threads < 1) threads = 1;
+        X = alto_gen::poisson_tensor((int)dims.size(), dims.data(), edges, rank, threads);
+        if (save) {
+            write_tensor(export_name(X, false).c_str(), X, false);
+            write_tensor(export_name(X, true).c_str(), X, true);
+        }
     }
     const char* base = strrchr(argv[0], '/');
     base = base ? base + 1 : argv[0];

@@ -72,6 +72,17 @@ def test_cli_check_bench_and_run(tmp_path):
         runs.append([l.split("\t")[1] for l in r.stdout.splitlines() if l.startswith("it: ")])
     assert (tmp_path / "3D_4000.bin").exists()
     assert len(runs[0]) == 4 and runs[0] == runs[1]
+    # -d / -s: the reference's Poisson generator (one thread), then CP-ALS
+    from oracle.poisson_gen import poisson_tensor
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([os.path.join(BIN, "cpd64"), "-d", "30,20,10", "-s", "0.05", "-r", "4", "-m", "3", "-x", "1",
+                        "-f", "1"], cwd=tmp_path, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Generating a sparse tensor: 30x20x10, rank = 4 ..." in r.stdout
+    assert len([l for l in r.stdout.splitlines() if l.startswith("it: ")]) == 3
+    ci, vo = poisson_tensor([30, 20, 10], 300, 4, 1)
+    d, cidx, vals = _read_bin(tmp_path / f"3D_{len(vo)}.bin")
+    assert d == [30, 20, 10] and np.array_equal(cidx, np.array(ci, np.uint64)) and np.array_equal(vals, vo)
 
 
 _IO_PROG = r'''
@@ -156,3 +167,60 @@ def test_tensor_io_formats(tmp_path):
     r = subprocess.run([str(exe), "x", str(tmp_path / "g.bin")], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode == 0 and r.stdout.strip() == "3D_3.tns"
     assert (tmp_path / "3D_3.tns").read_text().splitlines() == ["1 2 3 1.5", "4 1 2 0.125", "2 7 1 3"]
+
+
+_GEN_PROG = r'''
+#include "poisson_gen.hpp"
+#include "tensor_io.hpp"
+// edges rank threads dims...: the -d / -s generator, written as g.bin
+int main(int argc, char** argv) {
+    std::vector<IType> dims;
+    for (int i = 4; i < argc; ++i) dims.push_back(atoll(argv[i]));
+    SparseTensor* X = alto_gen::poisson_tensor((int)dims.size(), dims.data(), atoll(argv[1]), atoll(argv[2]),
+                                               atoi(argv[3]));
+    write_tensor("g.bin", X, true);
+    return 0;
+}
+'''
+
+
+def _read_bin(path):
+    import struct
+    b = open(path, "rb").read()
+    nm = struct.unpack("<Q", b[:8])[0] & 0xFFFFFFFF
+    dims = np.frombuffer(b[8:8 + 8 * nm], np.uint64)
+    nnz = struct.unpack("<Q", b[8 + 8 * nm:16 + 8 * nm])[0]
+    off = 16 + 8 * nm
+    cidx = np.frombuffer(b[off:off + 8 * nm * nnz], np.uint64).reshape(nm, nnz)
+    vals = np.frombuffer(b[off + 8 * nm * nnz:], np.float64)
+    return [int(d) for d in dims], cidx, vals
+
+
+@pytest.mark.parametrize("dims,edges,rank,threads", [([30, 20, 10], 300, 4, 1), ([40, 35, 30, 25], 2000, 5, 3),
+                                                     ([60, 50, 40], 120000, 3, 2)])
+def test_poisson_generator_matches_oracle(tmp_path, dims, edges, rank, threads):
+    """The -d / -s generator equals the oracle's restatement of the
+    reference's (same MT19937-64 streams per thread, glibc rand(), batches of
+    100000 draws) bit for bit; the counts add up to the draws, coordinates
+    are in range and sorted with the last mode most significant."""
+    from oracle.poisson_gen import poisson_tensor
+    src = tmp_path / "g.cpp"
+    src.write_text(_GEN_PROG)
+    exe = tmp_path / "g"
+    r = subprocess.run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
+                        "-I" + os.path.join(ROOT, "alto_amd", "cli"), str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(edges), str(rank), str(threads)] + [str(d) for d in dims], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("Generating a sparse tensor: " + "x".join(map(str, dims)) + f", rank = {rank} ...")
+    d, cidx, vals = _read_bin(tmp_path / "g.bin")
+    ci, vo = poisson_tensor(dims, edges, rank, threads)
+    assert d == dims
+    np.testing.assert_array_equal(cidx, np.array(ci, np.uint64))
+    np.testing.assert_array_equal(vals, np.array(vo))
+    assert vals.sum() == edges and (vals >= 1).all()
+    assert all((cidx[m] < dims[m]).all() for m in range(len(dims)))
+    key = [tuple(int(cidx[m, i]) for m in reversed(range(len(dims)))) for i in range(cidx.shape[1])]
+    assert all(a < b for a, b in zip(key, key[1:]))

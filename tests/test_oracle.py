@@ -347,3 +347,19 @@ def test_zipf_generator_shape():
     cidx, vals = zipf_tensor([300, 200, 500], 5000, 3)
     assert cidx.shape == (3, 5000) and np.all(vals > 0)
     assert len({tuple(c) for c in cidx.T.tolist()}) == 5000
+
+
+def test_poisson_oracle_mt19937_64_known_answer():
+    """The oracle generator's MT19937-64 is the published one: the C++
+    standard fixes the 10000th draw of std::mt19937_64 from its default seed
+    5489 at 9981545732273789042; its uniform draws lie in (0, 1)."""
+    from oracle.poisson_gen import MT64, Streams, binning
+    g = MT64(5489)
+    for _ in range(9999):
+        g()
+    assert g() == 9981545732273789042
+    u = Streams(2).uniform(1001, 0.0, 1.0)
+    assert all(0.0 < x < 1.0 for x in u)
+    # binning: the last edge below the draw; a draw on an edge takes the bin below
+    bins = [0.0, 0.25, 0.5, 1.0]
+    assert [binning(x, bins) for x in (0.1, 0.25, 0.3, 0.75, 1.0)] == [0, 1, 1, 2, 2]
