@@ -112,14 +112,23 @@ def test_mttkrp_fp32_bf16_error(R):
 
 @pytest.mark.parametrize("case", CASES[:6], ids=lambda c: f"{len(c['dims'])}m_{c['nnz']}")
 @pytest.mark.parametrize("R", [7, 10, 16, 32, 100])
-def test_phi_fp64(case, R):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_phi_fp64(case, R, variant):
+    """Both Phi kernels (runtime batch loop / compile-time batch with DPP
+    broadcasts) match the oracle for 2- to 5-mode tensors and 64- / 128-bit
+    keys."""
+    import alto_amd as A
     cidx, vals, g, o = _pair(case)
     rng = np.random.default_rng(R + 1)
     U = [rng.random((d, R)) + 0.05 for d in case["dims"]]
-    for mode in range(len(case["dims"])):
-        got = g.phi(mode, U, 1e-10)
-        want = o.phi(mode, U, 1e-10)
-        np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * max(np.abs(want).max(), 1.0))
+    A.check(A.lib().alto_tune_set(11, variant), "tune_set")
+    try:
+        for mode in range(len(case["dims"])):
+            got = g.phi(mode, U, 1e-10)
+            want = o.phi(mode, U, 1e-10)
+            np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-12 * max(np.abs(want).max(), 1.0))
+    finally:
+        A.lib().alto_tune_set(11, 1)
 
 
 def test_mttkrp_accumulate_and_zipf_heavy_rows():
@@ -145,7 +154,7 @@ def test_mttkrp_accumulate_and_zipf_heavy_rows():
 def tune_reset():
     A = _alto()
     yield A
-    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 32768), (10, 16)]:
+    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 32768), (10, 16), (11, 1)]:
         A.check(A.lib().alto_tune_set(param, value), "tune_set")
 
 
