@@ -104,6 +104,8 @@ def ref(W: int = 1):
         L.ref_create.restype = C.c_void_p
         L.ref_create.argtypes = [C.c_int, u64p, C.c_uint64, u64p, f64p, C.c_int]
         L.ref_destroy.argtypes = [C.c_void_p]
+        L.ref_wrap.restype = C.c_void_p
+        L.ref_wrap.argtypes = [C.c_int, u64p, C.c_uint64, u64p, u64p, u64p, f64p, C.c_int, u64p, u64p, u64p, u64p]
         L.ref_get.argtypes = [C.c_void_p, u64p, u64p, f64p, u64p, u64p, u64p, C.POINTER(C.c_int), u64p]
         L.ref_mttkrp_par.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_uint64]
         L.ref_mttkrp_seq.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.c_uint64]
@@ -262,6 +264,22 @@ class RefAlto:
         vals = np.ascontiguousarray(vals, dtype=np.float64)
         self.nnz = len(vals)
         self.h = self.L.ref_create(self.nmode, self.dims, self.nnz, cidx, vals, nprtn)
+
+    @classmethod
+    def wrap(cls, dims, W, nprtn, masks, alto_mask, idx, vals, prtn_ptr, intervals, cr_masks, alto_cr_mask):
+        """The reference's AltoTensor over a given ALTO state (no create_alto)."""
+        obj = cls.__new__(cls)
+        obj.L = ref(W)
+        obj.W = W
+        obj.dims = np.ascontiguousarray(dims, dtype=np.uint64)
+        obj.nmode = len(obj.dims)
+        obj.nprtn = nprtn
+        obj.nnz = len(vals)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.uint64).reshape(-1)
+        obj.h = obj.L.ref_wrap(obj.nmode, obj.dims, obj.nnz, c(masks), c(alto_mask), c(idx),
+                               np.ascontiguousarray(vals, dtype=np.float64), nprtn, c(prtn_ptr), c(intervals),
+                               c(cr_masks), c(alto_cr_mask))
+        return obj
 
     def __del__(self):
         if getattr(self, "h", None):

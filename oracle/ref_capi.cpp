@@ -4,6 +4,7 @@
 // tests can compare the oracle and the product against the real reference.
 // Nothing here is shipped or called by the product.
 #include <stdint.h>
+#include <string.h>
 #include <typeinfo>
 #include <vector>
 
@@ -26,6 +27,15 @@ static void put_lit(uint64_t *dst, LIT v)
 #endif
 }
 
+static LIT get_lit(const uint64_t *src)
+{
+#if W == 2
+    return (LIT)src[0] | ((LIT)src[1] << 64);
+#else
+    return (LIT)src[0];
+#endif
+}
+
 extern "C" {
 
 int ref_lit_words(void) { return W; }
@@ -43,6 +53,47 @@ void *ref_create(int nmodes, const uint64_t *dims, uint64_t nnz, const uint64_t 
     X.vals = (FType *)vals;
     AltoTensor<LIT> *at = nullptr;
     create_alto(&X, &at, nprtn);
+    return at;
+}
+
+// The reference's AltoTensor filled from an ALTO state produced elsewhere
+// (the GPU engine's export, bit-identical to create_alto's), so that the
+// reference's mttkrp_alto_par can be timed on a large tensor without its
+// CPU sort; freed by ref_destroy (destroy_alto).
+void *ref_wrap(int nmodes, const uint64_t *dims, uint64_t nnz, const uint64_t *masks, const uint64_t *alto_mask,
+               const uint64_t *idx, const double *vals, int nprtn, const uint64_t *prtn_ptr,
+               const uint64_t *intervals, const uint64_t *cr_masks, const uint64_t *alto_cr_mask)
+{
+    AltoTensor<LIT> *at = (AltoTensor<LIT> *)AlignedMalloc(sizeof(AltoTensor<LIT>));
+    memset((void *)at, 0, sizeof(AltoTensor<LIT>));
+    at->nmode = nmodes;
+    at->nprtn = nprtn;
+    at->nnz = nnz;
+    at->dims = (IType *)AlignedMalloc(nmodes * sizeof(IType));
+    at->mode_masks = (LIT *)AlignedMalloc(nmodes * sizeof(LIT));
+    at->cr_masks = (LIT *)AlignedMalloc(nmodes * sizeof(LIT));
+    for (int n = 0; n < nmodes; ++n) {
+        at->dims[n] = dims[n];
+        at->mode_masks[n] = get_lit(masks + n * W);
+        at->cr_masks[n] = get_lit(cr_masks + n * W);
+    }
+    at->alto_mask = get_lit(alto_mask);
+    at->alto_cr_mask = get_lit(alto_cr_mask);
+    at->idx = (LIT *)AlignedMalloc(nnz * sizeof(LIT));
+    at->vals = (ValType *)AlignedMalloc(nnz * sizeof(ValType));
+#pragma omp parallel for
+    for (uint64_t i = 0; i < nnz; ++i) {
+        at->idx[i] = get_lit(idx + i * W);
+        at->vals[i] = (ValType)vals[i];
+    }
+    at->prtn_ptr = (IType *)AlignedMalloc((nprtn + 1) * sizeof(IType));
+    for (int p = 0; p <= nprtn; ++p) at->prtn_ptr[p] = prtn_ptr[p];
+    at->prtn_intervals = (Interval *)AlignedMalloc((size_t)nprtn * nmodes * sizeof(Interval));
+    for (int k = 0; k < nprtn * nmodes; ++k) {
+        at->prtn_intervals[k].start = intervals[2 * k];
+        at->prtn_intervals[k].stop = intervals[2 * k + 1];
+    }
+    at->is_oidx = false;
     return at;
 }
 

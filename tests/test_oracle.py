@@ -363,3 +363,25 @@ def test_poisson_oracle_mt19937_64_known_answer():
     # binning: the last edge below the draw; a draw on an edge takes the bin below
     bins = [0.0, 0.25, 0.5, 1.0]
     assert [binning(x, bins) for x in (0.1, 0.25, 0.3, 0.75, 1.0)] == [0, 1, 1, 2, 2]
+
+
+@pytest.mark.skipif(not oracle.ref_available(1), reason="reference harness not built")
+@pytest.mark.parametrize("dims,nnz,W", [([300, 200, 100], 20000, 1), ([2**14, 2**13, 2**13, 2**12, 2**12], 8000, 2)])
+def test_reference_wrap_equals_create(dims, nnz, W):
+    """The reference's AltoTensor filled from an ALTO state (the path the
+    bench uses to time the reference's MTTKRP on the GPU's export) gives the
+    MTTKRPs of the one create_alto built."""
+    if not oracle.ref_available(W):
+        pytest.skip("reference harness not built")
+    cidx, vals = uniform_tensor(dims, nnz, 13)
+    r = oracle.RefAlto(dims, cidx, vals, W=W, nprtn=8)
+    st = r.state()
+    masks, cr = st["masks"], st["cr_masks"]
+    alto_mask = np.bitwise_or.reduce(masks, axis=0)
+    alto_cr = np.bitwise_or.reduce(cr, axis=0)
+    w = oracle.RefAlto.wrap(dims, W, 8, masks, alto_mask, st["idx"], st["vals"], st["prtn_ptr"], st["intervals"], cr,
+                            alto_cr)
+    rng = np.random.default_rng(2)
+    U = [rng.random((d, 16)) for d in dims]
+    for mode in range(len(dims)):
+        np.testing.assert_allclose(w.mttkrp(mode, U), r.mttkrp(mode, U), rtol=1e-13, atol=1e-13)
