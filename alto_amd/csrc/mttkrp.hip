@@ -494,8 +494,12 @@ int alto_mttkrp_autotune(alto_tensor_t* t, uint64_t rank, int precision, const v
         st = time_cfg(mode, -2, 0, &def_ms);
         int bv = -1, bs = 0;
         best_ms = def_ms;
+        // segment lengths within 4x of the size rule's choice (the others
+        // measured slower on every tensor tried, up to 10x on large ones)
+        const int h = tuned_seg(t->nnz, group_lanes(R), precision == ALTO_PREC_F64 ? 8 : 4);
         for (int v : variants)
             for (int sg : segs) {
+                if (sg * 4 < h || sg > h * 4) continue;
                 if (st != ALTO_OK) break;
                 double ms = 0;
                 st = time_cfg(mode, v, sg, &ms);

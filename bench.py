@@ -246,11 +246,14 @@ def apr_leg():
     t = time.perf_counter()
     r = T.cp_apr(U, lam, 30, 10, epsilon=0.0, eps_div_zero=0.0, dobench=True)
     dt = time.perf_counter() - t
-    phi_bytes = nnz * (16 + 8) + sum(d * 10 * 8 for d in dims) + 0  # + I_n R 8 per mode (averaged below)
+    # algorithmic bytes per Phi: the stream as stored (packed coordinates of
+    # 8 B per 64 index bits, 8 B value) + every factor once + Phi once
+    # (the output averaged over the modes)
+    s_pk = 8 if T.info().alto_bits <= 64 else 16
     phi_calls = r["total_inner"]
     per_phi = r["t_phi"] / max(phi_calls, 1)
     avg_out = sum(d * 10 * 8 for d in dims) / len(dims)
-    phi_gbs = (nnz * (16 + 8) + sum(d * 10 * 8 for d in dims) + avg_out) / per_phi / 1e9
+    phi_gbs = (nnz * (s_pk + 8) + sum(d * 10 * 8 for d in dims) + avg_out) / per_phi / 1e9
     log(f"cp-apr: {dt:.2f}s total, {phi_calls} Phi calls, {per_phi * 1e3:.3f} ms/Phi ({phi_gbs:.0f} GB/s)")
     del T
     torch.cuda.empty_cache()

@@ -141,7 +141,8 @@ int alto_tune_set(int param, int value);
 /* Autotuner (no reference counterpart; the reference fixes its shapes at
  * compile time, config.mk:25-26): for every mode of `t`, times `reps`
  * MTTKRPs of rank `rank` in `precision` for each lane mapping (0, 1, 2) and
- * segment length (16 .. 4096) against the defaults, and keeps the fastest
+ * segment length (powers of two from 16 to 4096, within 4x of the default
+ * for the tensor's size) against the defaults, and keeps the fastest
  * shape on the tensor; later MTTKRPs of that (mode, rank, precision) on `t`
  * -- CP-ALS sessions included -- use it while ALTO_TUNE_SEG and
  * ALTO_TUNE_MTTKRP_VARIANT are at their defaults.  `factors` are device
