@@ -27,7 +27,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <algorithm>
 #include <random>
+#include <utility>
 #include <vector>
 
 #include "alto/alto.hpp"
