@@ -251,6 +251,59 @@ def mttkrp_coo(dims, cidx, vals, factors, mode):
     return out
 
 
+def cpd_coo(dims, cidx, vals, factors, lam, max_iters, eps=1e-5):
+    """The reference's COO CP-ALS, ``cpd`` (include/cpd.hpp:274-371) with
+    ``cpd_fit`` (:373-455), restated with numpy / scipy: an independent
+    check of the ALTO-based ``OracleAlto.cpd_als`` (a different MTTKRP
+    traversal and a LAPACK solve).  Per iteration and mode: U_j = MTTKRP_j
+    V^-1 with V = prod of the other modes' Grams (Cholesky; least squares
+    when it fails, include/cpd.hpp:546-646), columns normalised (2-norm in
+    the first iteration, max(1, max_i U) after, src/kruskal_model.cpp:
+    224-341), Gram updated; then the fit.  Factors and lambda are updated in
+    place; returns (iterations, fits)."""
+    import scipy.linalg
+
+    N = len(dims)
+    U = factors
+    grams = [u.T @ u for u in U]
+    normX = float(np.dot(vals, vals))
+    fits, prev = [], 0.0
+    scratch = None
+    for it in range(max_iters):
+        for j in range(N):
+            Mj = mttkrp_coo(dims, cidx, vals, U, j)
+            if j == N - 1:
+                scratch = Mj.copy()
+            V = np.ones_like(grams[0])
+            for m in range(N):
+                if m != j:
+                    V *= grams[m]
+            try:
+                U[j][:] = scipy.linalg.cho_solve(scipy.linalg.cho_factor(V, lower=True), Mj.T).T
+            except np.linalg.LinAlgError:
+                U[j][:] = scipy.linalg.lstsq(V, Mj.T, lapack_driver="gelsy")[0].T
+            if it == 0:
+                lam[:] = np.sqrt((U[j] ** 2).sum(axis=0))
+            else:
+                lam[:] = np.maximum(1.0, np.maximum(U[j].max(axis=0), 0.0))
+            U[j] /= lam
+            grams[j] = U[j].T @ U[j]
+        inner = float(((U[N - 1] * scratch).sum(axis=0) * lam).sum())
+        tg = np.outer(lam, lam)
+        for g in grams:
+            tg = tg * g
+        normU = abs(float(np.tril(tg, -1).sum() * 2 + np.trace(tg)))
+        res = normX + normU - 2 * inner
+        if res > 0.0:
+            res = np.sqrt(res)
+        fit = 1 - res / np.sqrt(normX)
+        fits.append(fit)
+        if it > 0 and abs(prev - fit) < eps:
+            break
+        prev = fit
+    return len(fits), np.array(fits)
+
+
 class RefAlto:
     """The reference's own create_alto / mttkrp_alto_par / CalculatePhi (oracle/_ref)."""
 

@@ -385,3 +385,22 @@ def test_reference_wrap_equals_create(dims, nnz, W):
     U = [rng.random((d, 16)) for d in dims]
     for mode in range(len(dims)):
         np.testing.assert_allclose(w.mttkrp(mode, U), r.mttkrp(mode, U), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("dims,nnz,R", [([40, 50, 60], 6000, 8), ([30, 20, 10, 40], 8000, 16),
+                                        ([100, 90, 80, 70, 60], 20000, 10)])
+def test_cpd_als_matches_coo_cpd(dims, nnz, R):
+    """The oracle's ALTO CP-ALS equals the reference's COO cpd restated with
+    numpy / scipy (COO MTTKRP, LAPACK Cholesky solve): fit trace within 1e-10,
+    factors and lambda within 1e-8."""
+    cidx, vals = uniform_tensor(dims, nnz, 5)
+    o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
+    U1, l1 = oracle.random_init(dims, R, 3)
+    U2, l2 = oracle.random_init(dims, R, 3)
+    n1, f1 = o.cpd_als(U1, l1, 10, 0.0)
+    n2, f2 = oracle.cpd_coo(dims, cidx, vals, U2, l2, 10, 0.0)
+    assert n1 == n2 == 10
+    np.testing.assert_allclose(f1, f2, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(l1, l2, rtol=1e-8)
+    for a, b in zip(U1, U2):
+        np.testing.assert_allclose(a, b, rtol=1e-7, atol=1e-9 * np.abs(b).max())
