@@ -93,13 +93,19 @@ __global__ void k_kkt_partial(double* __restrict__ B, const double* __restrict__
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-__global__ void k_max_final(const double* __restrict__ part, int nb, double* __restrict__ out)
+// max of the nb per-block partials (one block of 256 threads)
+__global__ void __launch_bounds__(256) k_max_final(const double* __restrict__ part, int nb, double* __restrict__ out)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double m = 0;
-        for (int p = 0; p < nb; ++p) m = fmax(m, part[p]);
-        *out = m;
+    __shared__ double s[256];
+    double m = 0;
+    for (int p = threadIdx.x; p < nb; p += blockDim.x) m = fmax(m, part[p]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *out = s[0];
 }
 
 __global__ void k_mu(double* __restrict__ B, const double* __restrict__ Phi, uint64_t n)
@@ -298,9 +304,22 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
     double kkt_mode[ALTO_MAXN] = {0};
     int inner_per_mode[ALTO_MAXN] = {0};
     double t_phi = 0;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    // one event pair per inner iteration, read once per mode, so that the
+    // Phi timer does not stop the stream after every inner iteration
+    std::vector<hipEvent_t> ev0(max_inner, nullptr), ev1(max_inner, nullptr);
+    struct EventsGuard {
+        std::vector<hipEvent_t>* v[2];
+        ~EventsGuard()
+        {
+            for (auto* vv : v)
+                for (auto e : *vv)
+                    if (e) (void)hipEventDestroy(e);
+        }
+    } ev_guard{{&ev0, &ev1}};
+    for (int k = 0; k < max_inner; ++k) {
+        HIP_TRY(hipEventCreate(&ev0[k]));
+        HIP_TRY(hipEventCreate(&ev1[k]));
+    }
     const double* Uptr[ALTO_MAXN];
     for (int m = 0; m < N; ++m) Uptr[m] = c.U[m].as<double>();
     int iter;
@@ -321,16 +340,18 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
                 if (hit) n_viol[iter]++;
             }
             ALTO_TRY(redistribute(c, dim, n, s));
+            int timed = 0;
             for (int inner = 0; inner < max_inner; ++inner) {
                 n_inner[iter]++;
                 inner_per_mode[n]++;
-                HIP_TRY(hipEventRecord(e0, s));
+                HIP_TRY(hipEventRecord(ev0[inner], s));
                 ALTO_TRY(phi_device(t, n, R, Uptr, Ph, eps_div_zero, s, 0));
-                HIP_TRY(hipEventRecord(e1, s));
+                HIP_TRY(hipEventRecord(ev1[inner], s));
+                timed = inner + 1;
                 const unsigned nb = ew_blocks(ne);
                 hipLaunchKernelGGL(k_kkt_partial, dim3(nb), dim3(256), 0, s, B, Ph, ne, dobench ? 1 : 0,
                                    c.kpart.as<double>());
-                hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1), 0, s, c.kpart.as<double>(), (int)nb, c.kval.as<double>());
+                hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, c.kpart.as<double>(), (int)nb, c.kval.as<double>());
                 HIP_TRY(hipGetLastError());
                 double kv = 0;
                 if (!dobench || inner == max_inner - 1) {
@@ -338,14 +359,16 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
                     HIP_TRY(hipStreamSynchronize(s));
                     kkt_mode[n] = kv;
                 }
-                float ms = 0;
-                HIP_TRY(hipEventSynchronize(e1));
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                t_phi += ms * 1e-3;
                 if (dobench) continue;  // B *= Phi already applied
                 if (kv < epsilon) break;
                 hipLaunchKernelGGL(k_mu, dim3(nb), dim3(256), 0, s, B, Ph, ne);
                 HIP_TRY(hipGetLastError());
+            }
+            if (timed) HIP_TRY(hipEventSynchronize(ev1[timed - 1]));
+            for (int k = 0; k < timed; ++k) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, ev0[k], ev1[k]));
+                t_phi += ms * 1e-3;
             }
             ALTO_TRY(normalize_mode(c, dim, n, s));
         }
@@ -371,8 +394,6 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
                    n_viol[iter]);
     }
     HIP_TRY(hipStreamSynchronize(s));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     const double t_apr = now_s() - ta0;
 
     const double tq0 = now_s();
