@@ -126,11 +126,12 @@ def test_cpd_als_convergence_stop_and_rank1_recovery():
     assert n == no < 50
 
 
+@pytest.mark.parametrize("dims,nnz,R", [([300, 200, 400], 40000, 16), ([31, 45, 27], 3000, 7)])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-4), (2, 1e-3)])
-def test_cpd_als_reduced_precision_fit(prec, tol):
-    dims = [300, 200, 400]
-    cidx, vals = zipf_tensor(dims, 40000, 4, s=1.1)
-    R = 16
+def test_cpd_als_reduced_precision_fit(prec, tol, dims, nnz, R):
+    """Reduced-precision CP-ALS against the FP64 oracle; rank 7 with odd
+    dims leaves FP32 buffers that are not a multiple of 16 bytes."""
+    cidx, vals = zipf_tensor(dims, nnz, 4, s=1.1)
     n, fit, U, lam = _als(dims, cidx, vals, R, 15, prec=prec)
     o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
     Uo, lo = oracle.random_init(dims, R, 3)
@@ -181,3 +182,23 @@ def test_graph_replay_matches_eager(prec, tol):
     Ue, le, _, _ = eager.get()
     Ug, lg, _, _ = s.get()
     np.testing.assert_allclose(lg, le, rtol=1e3 * tol)
+
+
+def test_cpd_fit_of_returned_model_equals_last_fit():
+    """cpd_fit_alto on the model CP-ALS returned (alto_cpd_fit: the inner
+    product from the model's own MTTKRP) equals the fit of the last
+    iteration (computed inside the solve) and the oracle's cpd_fit_alto."""
+    A = _A()
+    dims = [60, 50, 40]
+    cidx, vals = uniform_tensor(dims, 6000, 41)
+    R = 8
+    n, fits, U, lam = _als(dims, cidx, vals, R, 6)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    f = ctypes.c_double(0)
+    A.check(A.lib().alto_cpd_fit(g.h, R, A._ptr_array([u.ctypes.data for u in U]), lam.ctypes.data, ctypes.byref(f)),
+            "cpd_fit")
+    np.testing.assert_allclose(f.value, fits[-1], rtol=1e-9, atol=1e-12)
+    o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
+    Uo, lo = oracle.random_init(dims, R, 3)
+    no, fo = o.cpd_als(Uo, lo, 6, 0.0)
+    np.testing.assert_allclose(f.value, fo[-1], rtol=1e-9, atol=1e-12)
