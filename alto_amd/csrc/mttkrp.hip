@@ -57,6 +57,7 @@ static int g_hot = 0;
 static int g_hot_block = 512;
 static int g_hot_lds = 32768;
 static int g_narrow_max = 16;
+static int g_narrow_small = 1;
 // Phi kernel: 0 = runtime batch loop with LDS permutes, 1 = compile-time
 // batch with DPP broadcasts (5% faster on the 120M-count CP-APR workload)
 static int g_phi_variant = 1;
@@ -346,6 +347,7 @@ static int mttkrp_device_shape(alto_tensor* t, int mode, int R, const void* cons
     k.hot_block = g_hot_block;
     k.hot_lds = g_hot_lds;
     k.narrow_max = g_narrow_max;
+    k.narrow_small = g_narrow_small;
     if (precision == ALTO_PREC_F64) {
         if (!accumulate) ALTO_TRY(fill_zero((double*)out, nout, s));
         return launch_mttkrp_f64(t, mode, R, (const double* const*)U, t->ms[mode].val64.as<double>(), (double*)out, seg,
@@ -439,6 +441,10 @@ int alto_tune_set(int param, int value)
     if (param == ALTO_TUNE_MTTKRP_VARIANT) {
         if (value < -1 || value > 2) return ALTO_ERR_ARG;
         g_variant = value;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_NARROW_SMALL) {
+        g_narrow_small = value ? 1 : 0;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_PHI_VARIANT) {

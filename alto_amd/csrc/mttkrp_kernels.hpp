@@ -20,6 +20,7 @@ struct MttkrpTune {
     int hot_block = 512;    // threads per workgroup of the ranked-stream kernel
     int hot_lds = 49152;    // LDS bytes per workgroup for hot rows
     int narrow_max = 16;    // FP32 / BF16 ranks that use two columns per lane
+    int narrow_small = 1;   // ... and every rank below 4M nonzeros
 };
 
 // Column mapping of a lane group: strided (lane gl holds c = gl + q*G, so
@@ -489,8 +490,13 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     // holds at most 16 groups (with 4 columns per lane it would hold up to
     // 32, and each row change of any group costs the wave a round of
     // atomics); measured 1.3x faster at ranks 8 and 16
+    // Below 4M nonzeros (flushes dominate, launches are latency-bound) every
+    // rank takes two columns per lane, as FP64 does with its 16-byte lanes:
+    // 4 groups per wave at rank 64 instead of 2 made FP32 ~40% slower than
+    // FP64 there.
     constexpr bool NARROW_OK = !std::is_same<FT, double>::value;
-    if (v == 2 && NARROW_OK && R <= k.narrow_max && !split && a.nother >= 1 && rows_aligned(a, R, 2 * (int)sizeof(FT))) {
+    const bool narrow_rank = R <= k.narrow_max || (k.narrow_small && t->nnz < (4ull << 20) && R <= 128);
+    if (v == 2 && NARROW_OK && narrow_rank && !split && a.nother >= 1 && rows_aligned(a, R, 2 * (int)sizeof(FT))) {
         int G = 1;
         while (G * 2 < R) G <<= 1;
         if (seg <= 0) seg = tuned_seg(t->nnz, G, (int)sizeof(AT));
@@ -516,6 +522,9 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
         KNG_(2);
         KNG_(4);
         KNG_(8);
+        KNG_(16);
+        KNG_(32);
+        KNG_(64);
 #undef KNG_
 #undef KN_
         HIP_TRY(hipGetLastError());

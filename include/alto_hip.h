@@ -132,10 +132,12 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * ALTO_TUNE_NARROW_RANK: FP32 / BF16 ranks up to this value (0..16, default
  * 16) use two columns per lane in the vector-lane mapping.
  * ALTO_TUNE_PHI_VARIANT: CP-APR Phi kernel, 0 = runtime batch loop with LDS
- * permutes, 1 = compile-time batch with DPP broadcasts (default). */
+ * permutes, 1 = compile-time batch with DPP broadcasts (default).
+ * ALTO_TUNE_NARROW_SMALL: 1 (default) = below 4M nonzeros every FP32 / BF16
+ * rank uses two columns per lane. */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
        ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9,
-       ALTO_TUNE_NARROW_RANK = 10, ALTO_TUNE_PHI_VARIANT = 11 };
+       ALTO_TUNE_NARROW_RANK = 10, ALTO_TUNE_PHI_VARIANT = 11, ALTO_TUNE_NARROW_SMALL = 12 };
 int alto_tune_set(int param, int value);
 
 /* Autotuner (no reference counterpart; the reference fixes its shapes at

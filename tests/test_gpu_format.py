@@ -154,7 +154,7 @@ def test_mttkrp_accumulate_and_zipf_heavy_rows():
 def tune_reset():
     A = _alto()
     yield A
-    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 32768), (10, 16), (11, 1)]:
+    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 32768), (10, 16), (11, 1), (12, 1)]:
         A.check(A.lib().alto_tune_set(param, value), "tune_set")
 
 
@@ -207,6 +207,26 @@ def test_mttkrp_segment_lengths_keep_results(tune_reset, seg, dims):
             want = o.mttkrp(mode, U)
             np.testing.assert_allclose(g.mttkrp(mode, U), want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
             assert _rel(g.mttkrp(mode, U, A.PREC_F32).astype(np.float64), want) < 1e-4
+
+
+@pytest.mark.parametrize("narrow_small", [0, 1])
+@pytest.mark.parametrize("dims", [[300, 40, 700, 25], [500, 3000]], ids=lambda d: f"{len(d)}m")
+def test_mttkrp_two_columns_per_lane_small_tensors(tune_reset, narrow_small, dims):
+    """FP32 / BF16 MTTKRPs of every rank with two columns per lane (the
+    small-tensor default) or the wider lanes match numpy on the stored
+    factors."""
+    A = tune_reset
+    A.check(A.lib().alto_tune_set(12, narrow_small), "tune_set")
+    cidx, vals = zipf_tensor(dims, 20000, 31)
+    g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+    for R in (24, 32, 64, 100, 128):
+        rng = np.random.default_rng(R)
+        U = [rng.random((d, R)) for d in dims]
+        for mode in range(len(dims)):
+            U32 = [u.astype(np.float32).astype(np.float64) for u in U]
+            assert _rel(g.mttkrp(mode, U, A.PREC_F32).astype(np.float64), mttkrp_numpy(cidx, vals, U32, mode)) < 1e-4
+            Ub = [A.from_bf16_bits(A.to_bf16_bits(u)).astype(np.float64) for u in U]
+            assert _rel(g.mttkrp(mode, U, A.PREC_BF16).astype(np.float64), mttkrp_numpy(cidx, vals, Ub, mode)) < 1e-4
 
 
 @pytest.mark.parametrize("prec", [0, 1, 2])
