@@ -10,7 +10,8 @@
 //    from a Jacobi eigen-decomposition.
 //    Reference: pseudo_inverse (include/cpd.hpp:546-646).
 //  - Row solves U_n[i,:] = M[i,:] V^-1 as a GEMM with the explicit inverse
-//    (or the pseudo-inverse), MFMA FP64, fused with the per-block column norms.
+//    (or the pseudo-inverse), MFMA FP64 (FP32 for FP32 / BF16 models), fused
+//    with the per-block column norms.
 //  - Column normalisation (2-norm / max-norm), the fit, ||X||^2.
 //    Reference: KruskalModelNorm (src/kruskal_model.cpp:224-341),
 //    cpd_fit_alto (include/cpd.hpp:189-271).
@@ -564,7 +565,8 @@ __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ g
 
 // ------------------------------------------------------------------ row solves
 // U[i,:] = M[i,:] W with W = V^-1 (or V^+), as a GEMM on the matrix cores
-// (v_mfma_f64_16x16x4f64): each wave takes 16 rows at a time; per k-step of
+// (v_mfma_f64_16x16x4f64; v_mfma_f32_16x16x4f32 for FP32 / BF16 models):
+// each wave takes 16 rows at a time; per k-step of
 // 4 it loads one M element per lane (A operand: lane l holds M[row0 + l%16]
 // [k0 + l/16]) and multiplies it into every 16-column tile of the output
 // (B operand from the LDS copy of W: lane l holds W[k0 + l/16][c0 + l%16]).
@@ -579,10 +581,16 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
                                                UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
                                                double* __restrict__ pin)
 {
+    // FP32 / BF16 models (M and U in FP32): the products on FP32 MFMA with
+    // the FP64 inverse rounded to FP32; FP64 models: FP64 MFMA
+    constexpr bool F32 = std::is_same<MT, float>::value && std::is_same<UT, float>::value;
+    using mma_t = typename std::conditional<F32, float, double>::type;
+    using acc_t = typename std::conditional<F32, f4_t, d4_t>::type;
     constexpr int KS = 4 * T16;  // k-steps of 4 covering 16 T16 >= R columns
-    extern __shared__ double sW[];  // W, row stride WS
+    extern __shared__ double sWd[];  // W, row stride WS
+    mma_t* sW = reinterpret_cast<mma_t*>(sWd);
     const int WS = R + 2;
-    for (int k = threadIdx.x; k < R * R; k += blockDim.x) sW[(k / R) * WS + k % R] = Wg[k];
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) sW[(k / R) * WS + k % R] = (mma_t)Wg[k];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, kq = lane >> 4;
@@ -595,15 +603,15 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
     double in = 0;
     for (uint64_t row0 = ((uint64_t)blockIdx.x * 4 + wave) * 16; row0 < I; row0 += (uint64_t)gridDim.x * 64) {
         const uint64_t ra = row0 + r16;
-        double av[KS];
+        mma_t av[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int k = ks * 4 + kq;
-            av[ks] = (ra < I && k < R) ? (double)M[ra * R + k] : 0.0;
+            av[ks] = (ra < I && k < R) ? (mma_t)M[ra * R + k] : (mma_t)0;
         }
-        d4_t acc[T16];
+        acc_t acc[T16];
 #pragma unroll
-        for (int t = 0; t < T16; ++t) acc[t] = d4_t{0, 0, 0, 0};
+        for (int t = 0; t < T16; ++t) acc[t] = acc_t{0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             if (ks * 4 < R) {  // uniform
@@ -611,18 +619,22 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
 #pragma unroll
                 for (int t = 0; t < T16; ++t) {
                     const int c = t * 16 + r16;
-                    const double b = (k < R && c < R) ? sW[k * WS + c] : 0.0;
-                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], b, acc[t], 0, 0, 0);
+                    const mma_t b = (k < R && c < R) ? sW[k * WS + c] : (mma_t)0;
+                    if constexpr (F32)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[ks], b, acc[t], 0, 0, 0);
+                    else
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], b, acc[t], 0, 0, 0);
                 }
             }
         }
-        // register v of lane l holds C[l/16 + 4v][l%16]
+        // register v of lane l holds C[l/16 + 4v][l%16] (FP64) or
+        // C[4 (l/16) + v][l%16] (FP32)
 #pragma unroll
         for (int t = 0; t < T16; ++t) {
             const int c = t * 16 + r16;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const uint64_t r = row0 + kq + 4 * v;
+                const uint64_t r = row0 + (F32 ? 4 * kq + v : kq + 4 * v);
                 if (r < I && c < R) {
                     const UT x = (UT)acc[t][v];
                     U[r * R + c] = x;
