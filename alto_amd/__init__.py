@@ -84,6 +84,7 @@ def lib():
         "alto_pdep_keys": (i32, [vp, vp, u64, vp, vp]),
         "alto_pext_keys": (i32, [vp, vp, u64, vp, vp]),
         "alto_mttkrp": (i32, [vp, i32, u64, vp, vp, i32, i32, vp]),
+        "alto_mttkrp_batched": (i32, [i32, vp, vp, u64, vp, vp, i32, i32, vp]),
         "alto_phi": (i32, [vp, i32, u64, vp, vp, f64, vp]),
         "alto_cpd_als": (i32, [vp, u64, i32, f64, i32, vp, vp, vp, C.POINTER(i32), C.POINTER(CpdTimers), i32, vp]),
         "alto_cpd_fit": (i32, [vp, u64, vp, vp, C.POINTER(f64)]),
@@ -366,6 +367,38 @@ class CpdSession:
         if h and _lib is not None:
             _lib.alto_cpd_session_destroy(h)
             self.h = None
+
+
+def _factor_arrays(factors, precision):
+    if precision == PREC_F64:
+        return [DeviceArray.from_numpy(np.asarray(f, np.float64)) for f in factors]
+    if precision == PREC_F32:
+        return [DeviceArray.from_numpy(np.asarray(f, np.float32)) for f in factors]
+    return [DeviceArray.from_numpy(to_bf16_bits(f)) for f in factors]
+
+
+def mttkrp_batched(tensors, modes, factors, precision=PREC_F64):
+    """alto_mttkrp_batched on host arrays: job i is the mode-modes[i] MTTKRP
+    of tensors[i] (an AltoTensor) with factors[i] (its host factor list); all
+    jobs share the rank.  Returns the list of outputs."""
+    n = len(tensors)
+    if not (len(modes) == len(factors) == n):
+        raise ValueError("tensors, modes and factors differ in length")
+    R = factors[0][0].shape[1] if n else 1
+    keep, fptrs, outs = [], [], []
+    for t, m, fs in zip(tensors, modes, factors):
+        dev = _factor_arrays(fs, precision)
+        arr = _ptr_array([d.ptr for d in dev])
+        keep.append((dev, arr))
+        fptrs.append(C.cast(arr, C.c_void_p).value)
+        rows = int(t.dims[m]) if 0 <= m < len(t.dims) else 1  # a bad mode is the library's to reject
+        outs.append(DeviceArray((rows, R), np.float64 if precision == PREC_F64 else np.float32))
+    th = _ptr_array([t.h for t in tensors])
+    md = (C.c_int * n)(*modes)
+    check(lib().alto_mttkrp_batched(n, th, md, R, _ptr_array(fptrs), _ptr_array([o.ptr for o in outs]), precision,
+                                    0, None), "alto_mttkrp_batched")
+    synchronize()
+    return [o.numpy() for o in outs]
 
 
 def create_alto(dims, cidx, vals, nprtn=8, lit_bits=0, val_kind=VAL_F64):

@@ -540,6 +540,68 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
     return mttkrp_device(t, mode, (int)rank, factors, out, precision, accumulate, (hipStream_t)stream, 0);
 }
 
+int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* modes, uint64_t rank,
+                        const void* const* const* factors, void* const* outs, int precision, int accumulate,
+                        alto_stream_t stream)
+{
+    if (njobs < 0 || (njobs > 0 && (!tensors || !modes || !factors || !outs))) {
+        set_error("alto_mttkrp_batched: null argument");
+        return ALTO_ERR_ARG;
+    }
+    if (precision != ALTO_PREC_F64 && precision != ALTO_PREC_F32 && precision != ALTO_PREC_BF16) {
+        set_error("alto_mttkrp_batched: unknown precision %d", precision);
+        return ALTO_ERR_ARG;
+    }
+    const int R = (int)rank;
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < njobs; ++i) {
+        alto_tensor* t = tensors[i];
+        if (!t || !factors[i] || !outs[i]) {
+            set_error("alto_mttkrp_batched: null argument in job %d", i);
+            return ALTO_ERR_ARG;
+        }
+        if (modes[i] < 0 || modes[i] >= t->nmode || rank < 1 || rank > 128) {
+            set_error("alto_mttkrp_batched: job %d: mode %d / rank %llu out of range (rank 1..128)", i, modes[i],
+                      (unsigned long long)rank);
+            return ALTO_ERR_ARG;
+        }
+        for (int m = 0; m < t->nmode; ++m)
+            if (!factors[i][m]) {
+                set_error("alto_mttkrp_batched: job %d: null factor %d", i, m);
+                return ALTO_ERR_ARG;
+            }
+    }
+    MttkrpTune k;
+    k.variant = g_variant;
+    k.nt = g_nt;
+    k.hot_block = g_hot_block;
+    k.hot_lds = g_hot_lds;
+    k.narrow_max = g_narrow_max;
+    k.narrow_small = g_narrow_small;
+    std::vector<const void*> vals(njobs);
+    for (int i = 0; i < njobs; ++i) {
+        alto_tensor* t = tensors[i];
+        ALTO_TRY(mttkrp_prepare(t, R, precision, s));
+        const uint64_t nout = t->dims[modes[i]] * (uint64_t)R;
+        if (precision == ALTO_PREC_F64) {
+            vals[i] = t->ms[modes[i]].val64.as<double>();
+            if (!accumulate) ALTO_TRY(fill_zero((double*)outs[i], nout, s));
+        } else {
+            vals[i] = t->ms[modes[i]].val32.as<float>();
+            if (!accumulate) ALTO_TRY(fill_zero((float*)outs[i], nout, s));
+        }
+    }
+    if (njobs == 0) return ALTO_OK;
+    if (precision == ALTO_PREC_F64)
+        return launch_mttkrp_batch_f64(njobs, tensors, modes, R, (const double* const* const*)factors,
+                                       (const double* const*)vals.data(), (double* const*)outs, k, s);
+    if (precision == ALTO_PREC_F32)
+        return launch_mttkrp_batch_f32(njobs, tensors, modes, R, (const float* const* const*)factors,
+                                       (const float* const*)vals.data(), (float* const*)outs, k, s);
+    return launch_mttkrp_batch_bf16(njobs, tensors, modes, R, (const bf16_t* const* const*)factors,
+                                    (const float* const*)vals.data(), (float* const*)outs, k, s);
+}
+
 int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* factors, double* phi, double eps_div_zero,
              alto_stream_t stream)
 {

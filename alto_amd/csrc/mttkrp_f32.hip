@@ -9,4 +9,10 @@ int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const*
     return launch_mttkrp<float, float, float, float>(t, mode, R, U, val, out, seg, k, s);
 }
 
+int launch_mttkrp_batch_f32(int n, alto_tensor* const* ts, const int* modes, int R, const float* const* const* U,
+                            const float* const* vals, float* const* outs, const MttkrpTune& k, hipStream_t s)
+{
+    return launch_mttkrp_batch<float, float, float, float>(n, ts, modes, R, U, vals, outs, k, s);
+}
+
 }  // namespace alto

@@ -6,7 +6,9 @@
 // (include/alto.hpp:617-623, :1227-1318).
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
+#include <vector>
 
 #include "alto_internal.hpp"
 #include "kernels_common.hpp"
@@ -336,8 +338,11 @@ __device__ __forceinline__ void load16(P row, int gl, int R, AT (&x)[E])
 // mask of a hot-pattern group of a split stream: for other mode o with bit
 // o set every nonzero's rank is below hot_n[o] / R, and the row is read from
 // the LDS copy of those rows (staged once per workgroup).
-template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO, int HM>
-__global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
+// RTNO: NO is an upper bound and modes o >= a.nother are skipped at run time
+// (the multi-tensor launch, whose tensors differ in their number of modes).
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO, int HM, bool RTNO>
+__device__ __forceinline__ void mttkrp16_body(const StreamArgs<FT, VT, OT>& a, uint64_t block,
+                                              unsigned char* hot_smem)
 {
     // a batch is B = max(G, 4) nonzeros: each lane decodes NS = B / G of them
     // (slot k of lane l holds nonzero p + k*G + l); J = 4 of them are in
@@ -345,12 +350,8 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
     constexpr int NS = G < 4 ? 4 / G : 1;
     constexpr int B = G * NS;
     constexpr int J = 4;
-    extern __shared__ __align__(16) unsigned char hot_smem[];
-    if constexpr (HM != 0) stage_hot_rows(a, hot_smem);
     const int lane = threadIdx.x & 63;
     const int gl = lane & (G - 1);
-    const uint64_t block = xcd_block(a.nblocks_real);
-    if (block == ~0ull) return;
     const uint64_t group = (block * 256 + threadIdx.x) / G;
     const uint64_t s = group * a.seg;
     if (s >= a.nnz) return;
@@ -406,6 +407,8 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
             });
             static_for<0, NO>([&](auto oi) {
                 constexpr int o = decltype(oi)::value;
+                if constexpr (RTNO)
+                    if (o >= a.nother) return;  // uniform
                 const FT* __restrict__ U = a.Uo[o];
                 const lds_ptr<FT> H = (lds_ptr<FT>)(hot_smem + a.hot_base[o]);
                 static_for<0, J>([&](auto ji) {
@@ -450,6 +453,39 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
         });
     }
     if (cur != 0xffffffffu) flush_row<AT, OT, G, E, true>(a.out, cur, R, gl, acc);
+}
+
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NO, int HM>
+__global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
+{
+    extern __shared__ __align__(16) unsigned char hot_smem[];
+    if constexpr (HM != 0) stage_hot_rows(a, hot_smem);
+    const uint64_t block = xcd_block(a.nblocks_real);
+    if (block == ~0ull) return;
+    mttkrp16_body<FT, VT, AT, OT, G, E, PW, NO, HM, false>(a, block, hot_smem);
+}
+
+// Many MTTKRPs (different tensors, modes, sizes; one rank and lane shape) in
+// one launch: the table travels in the kernel arguments, and job j owns the
+// logical workgroups [start[j], start[j+1]) -- its own launch's grid, laid
+// end to end with the other jobs' and mapped XCD-aware as a whole.
+constexpr int MTTKRP_BATCH_MAX = 16;
+template <typename FT, typename VT, typename OT>
+struct BatchTable {
+    StreamArgs<FT, VT, OT> a[MTTKRP_BATCH_MAX];
+    uint32_t start[MTTKRP_BATCH_MAX + 1];
+    int n;
+};
+
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW>
+__global__ void __launch_bounds__(256) k_mttkrp16_batch(BatchTable<FT, VT, OT> tab)
+{
+    extern __shared__ __align__(16) unsigned char hot_smem[];
+    const uint64_t block = xcd_block(tab.start[tab.n]);
+    if (block == ~0ull) return;
+    int j = 0;
+    while (j + 1 < tab.n && block >= tab.start[j + 1]) ++j;  // uniform
+    mttkrp16_body<FT, VT, AT, OT, G, E, PW, ALTO_MAXN - 1, 0, true>(tab.a[j], block - tab.start[j], hot_smem);
 }
 
 // the vector-lane kernel needs factor rows aligned to the lane's load width
@@ -636,11 +672,119 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     return ALTO_OK;
 }
 
+// lanes per nonzero (G) and elements per lane (E) of the vector-lane kernel
+// for this tensor and mode, as launch_mttkrp picks them; false when the
+// launch would take another kernel (ranked streams, unaligned rows, other
+// mappings)
+template <typename FT, typename VT, typename OT>
+inline bool vector_lane_shape(const alto_tensor* t, int mode, int R, const StreamArgs<FT, VT, OT>& a,
+                              const MttkrpTune& k, int& G, int& E)
+{
+    constexpr int E16 = std::is_same<FT, bf16_t>::value ? 4 : 16 / sizeof(FT);
+    constexpr bool NARROW_OK = !std::is_same<FT, double>::value;
+    if (!(k.variant < 0 || k.variant == 2) || a.nother < 1 || t->ms[mode].ranked != 0) return false;
+    const bool narrow_rank = R <= k.narrow_max || (k.narrow_small && t->nnz < (4ull << 20) && R <= 128);
+    if (NARROW_OK && narrow_rank && rows_aligned(a, R, 2 * (int)sizeof(FT))) E = 2;
+    else if (rows_aligned(a, R, E16 * (int)sizeof(FT)) && R <= 64 * E16) E = E16;
+    else return false;
+    G = 1;
+    while (G * E < R) G <<= 1;
+    return true;
+}
+
+template <typename FT, typename VT, typename AT, typename OT, int E>
+inline void launch_batch_table(const BatchTable<FT, VT, OT>& tab, int G, int pw, dim3 grid, hipStream_t s)
+{
+#define KB_(GG)                                                                                              \
+    do {                                                                                                     \
+        if (pw == 1) hipLaunchKernelGGL((k_mttkrp16_batch<FT, VT, AT, OT, GG, E, 1>), grid, dim3(256), 0, s, tab); \
+        else hipLaunchKernelGGL((k_mttkrp16_batch<FT, VT, AT, OT, GG, E, 2>), grid, dim3(256), 0, s, tab);        \
+    } while (0)
+    switch (G) {
+    case 2: KB_(2); break;
+    case 4: KB_(4); break;
+    case 8: KB_(8); break;
+    case 16: KB_(16); break;
+    default: KB_(32); break;
+    }
+#undef KB_
+}
+
+// Job i: the MTTKRP of ts[i] along modes[i] with factors U[i] and values
+// vals[i] into outs[i] (cleared or accumulated by the caller).  Jobs of one
+// lane shape (G in 2..32, E, key width) share launches of up to
+// MTTKRP_BATCH_MAX jobs; the others are launched one by one through
+// launch_mttkrp.  Segment lengths follow the size rule of each tensor.
+template <typename FT, typename VT, typename AT, typename OT>
+int launch_mttkrp_batch(int n, alto_tensor* const* ts, const int* modes, int R, const FT* const* const* U,
+                        const VT* const* vals, OT* const* outs, const MttkrpTune& k, hipStream_t s)
+{
+    constexpr int E16 = std::is_same<FT, bf16_t>::value ? 4 : 16 / sizeof(FT);
+    struct Group {
+        int G, E, pw;
+        std::vector<StreamArgs<FT, VT, OT>> jobs;
+    };
+    std::vector<Group> groups;
+    for (int i = 0; i < n; ++i) {
+        const alto_tensor* t = ts[i];
+        StreamArgs<FT, VT, OT> a;
+        fill_stream_args(t, modes[i], R, 0, a);
+        ALTO_TRY(set_factors(t, modes[i], U[i], a, s));
+        a.val = vals[i];
+        a.out = outs[i];
+        int G = 0, E = 0;
+        if (!vector_lane_shape(t, modes[i], R, a, k, G, E) || G < 2 || G > 32) {
+            ALTO_TRY((launch_mttkrp<FT, VT, AT, OT>(t, modes[i], R, U[i], vals[i], outs[i], 0, k, s)));
+            continue;
+        }
+        a.seg = (uint64_t)tuned_seg(t->nnz, G, (int)sizeof(AT));
+        (void)stream_grid(t->nnz, (int)a.seg, G, a);
+        Group* g = nullptr;
+        for (auto& x : groups)
+            if (x.G == G && x.E == E && x.pw == t->pack.pw) g = &x;
+        if (!g) {
+            groups.push_back(Group{G, E, t->pack.pw, {}});
+            g = &groups.back();
+        }
+        g->jobs.push_back(a);
+    }
+    for (const auto& g : groups) {
+        for (size_t j0 = 0; j0 < g.jobs.size(); j0 += MTTKRP_BATCH_MAX) {
+            BatchTable<FT, VT, OT> tab;
+            tab.n = (int)std::min<size_t>(MTTKRP_BATCH_MAX, g.jobs.size() - j0);
+            uint64_t total = 0;
+            for (int j = 0; j < MTTKRP_BATCH_MAX; ++j) {
+                if (j < tab.n) tab.a[j] = g.jobs[j0 + j];
+                else tab.a[j] = g.jobs[j0];  // unused entries
+                tab.start[j] = (uint32_t)total;
+                if (j < tab.n) total += g.jobs[j0 + j].nblocks_real;
+            }
+            tab.start[MTTKRP_BATCH_MAX] = (uint32_t)total;
+            for (int j = tab.n; j <= MTTKRP_BATCH_MAX; ++j) tab.start[j] = (uint32_t)total;
+            if (total >= (1ull << 31)) {
+                set_error("mttkrp_batched: %llu workgroups in one launch", (unsigned long long)total);
+                return ALTO_ERR_UNSUPPORTED;
+            }
+            const dim3 grid((unsigned)((total + 7) / 8 * 8));
+            if (g.E == 2) launch_batch_table<FT, VT, AT, OT, 2>(tab, g.G, g.pw, grid, s);
+            else if constexpr (E16 != 2) launch_batch_table<FT, VT, AT, OT, E16>(tab, g.G, g.pw, grid, s);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    return ALTO_OK;
+}
+
 int launch_mttkrp_f64(const alto_tensor* t, int mode, int R, const double* const* U, const double* val, double* out,
                       int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const* U, const float* val, float* out,
                       int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_bf16(const alto_tensor* t, int mode, int R, const bf16_t* const* U, const float* val, float* out,
                        int seg, const MttkrpTune& k, hipStream_t s);
+int launch_mttkrp_batch_f64(int n, alto_tensor* const* ts, const int* modes, int R, const double* const* const* U,
+                            const double* const* vals, double* const* outs, const MttkrpTune& k, hipStream_t s);
+int launch_mttkrp_batch_f32(int n, alto_tensor* const* ts, const int* modes, int R, const float* const* const* U,
+                            const float* const* vals, float* const* outs, const MttkrpTune& k, hipStream_t s);
+int launch_mttkrp_batch_bf16(int n, alto_tensor* const* ts, const int* modes, int R, const bf16_t* const* const* U,
+                             const float* const* vals, float* const* outs, const MttkrpTune& k, hipStream_t s);
 
 }  // namespace alto

@@ -106,6 +106,21 @@ int alto_pext_keys(const alto_tensor_t* t, const uint64_t* keys, uint64_t n, uin
 int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* factors, void* out,
                 int precision, int accumulate, alto_stream_t stream);
 
+/* Many MTTKRPs in one launch (no reference counterpart: the reference runs
+ * one mttkrp_alto_par per call, include/alto.hpp:617-623; this is the
+ * multi-tensor form for batches of small decompositions).  Job i is
+ * alto_mttkrp(tensors[i], modes[i], rank, factors[i], outs[i], precision,
+ * accumulate): factors[i] holds that tensor's nmodes device pointers.  Jobs
+ * that take the vector-lane kernel with the same lanes per nonzero, elements
+ * per lane and key width share a launch of up to 16 jobs; each job keeps its
+ * own segment length and grid, laid end to end with the others'.  Other jobs
+ * are launched one by one.  Shapes chosen by alto_mttkrp_autotune are not
+ * used here.  Each output equals that job's alto_mttkrp up to the order of
+ * the floating-point atomic additions.  Asynchronous on `stream`. */
+int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* modes, uint64_t rank,
+                        const void* const* const* factors, void* const* outs, int precision, int accumulate,
+                        alto_stream_t stream);
+
 /* Tuning knobs of the stream kernels (no reference counterpart; the
  * reference specializes at compile time, config.mk:25-26).  ALTO_TUNE_SEG:
  * nonzeros walked by one lane group (0 = default).  ALTO_TUNE_CHUNK_LOG2:
