@@ -71,6 +71,8 @@ def main():
                     help="profiling driver: build the tensor, run REPS all-mode MTTKRPs, print nothing else")
     ap.add_argument("--copy-only", action="store_true",
                     help="profiling calibration: a 256 MiB device copy (for FETCH_SIZE / WRITE_SIZE scaling)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="profiling driver: only the 16-job batched MTTKRP comparison, its JSON on stdout")
     ap.add_argument("--serve-jobs", type=int, default=100,
                     help="jobs of the Poisson(50/s) serving stream to run (0 = skip; the full stream is 2000)")
     args = ap.parse_args()
@@ -93,6 +95,9 @@ def main():
         for _ in range(4):
             b.copy_(a)
         torch.cuda.synchronize()
+        return
+    if args.batched_only:
+        print(json.dumps(batched_leg()), flush=True)
         return
 
     t0 = time.time()
@@ -186,6 +191,7 @@ def main():
     serving = None
     if args.serve_jobs > 0:
         serving = serve_leg(args.serve_jobs)
+    batched = batched_leg() if args.serve_jobs > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -217,6 +223,7 @@ def main():
         "cpd_timers_s": tm,
         "precision_variants": variants,
         "serving": serving,
+        "batched_mttkrp": batched,
         "cp_apr": apr,
         "precision_study": prec_study,
         "cpu_baseline": cpu,
@@ -384,6 +391,78 @@ def serve_leg(n_jobs):
         out[name + "_saturated_speedup_vs_fp64"] = (out[name + "_batched_saturated"]["jobs_per_s"] /
                                                     out["fp64_per_job_saturated"]["jobs_per_s"])
     del jobs
+    torch.cuda.empty_cache()
+    return out
+
+
+def batched_leg(n_jobs=16, reps=50):
+    """Many small MTTKRPs: one alto_mttkrp_batched call against one
+    alto_mttkrp call per job, on one stream, timed with events on that stream
+    over `reps` back-to-back calls (host submission included).  16 tensors of
+    3-5 modes, dims log-uniform in [64, 2^16], nnz log-uniform in [1e4, 2e5],
+    Zipf(1.1), lognormal values; mode-0 MTTKRP accumulated into the output."""
+    import ctypes as C
+
+    import torch
+
+    import alto_amd as A
+    from alto_amd.gen import sparse_coo
+
+    rng = np.random.default_rng(11)
+    L = A.lib()
+    s = torch.cuda.current_stream()
+    tensors, keep = [], []
+    for k in range(n_jobs):
+        dims = [int(np.exp(rng.uniform(np.log(64), np.log(2 ** 16)))) for _ in range(3 + k % 3)]
+        nnz = int(np.exp(rng.uniform(np.log(1e4), np.log(2e5))))
+        coords, vals = sparse_coo(dims, nnz, seed=300 + k)
+        nnz = int(vals.numel())  # fewer when the coordinate space is small
+        coords = [c.to(torch.int64).contiguous() for c in coords]
+        t = A.AltoTensor.from_device(dims, [c.data_ptr() for c in coords], vals.data_ptr(), nnz)
+        torch.cuda.synchronize()
+        del coords, vals
+        tensors.append((t, dims, nnz))
+    out = {"jobs": n_jobs, "total_nnz": int(sum(x[2] for x in tensors)), "calls_timed": reps}
+    for R, prec, name in ((16, A.PREC_F64, "r16_fp64"), (32, A.PREC_F64, "r32_fp64"), (16, A.PREC_F32, "r16_fp32"),
+                          (32, A.PREC_F32, "r32_fp32")):
+        dt = torch.float64 if prec == A.PREC_F64 else torch.float32
+        fptrs, outs = [], []
+        for t, dims, _ in tensors:
+            fs = [torch.rand((d, R), dtype=dt, device="cuda") for d in dims]
+            arr = A._ptr_array([f.data_ptr() for f in fs])
+            o = torch.zeros((dims[0], R), dtype=dt, device="cuda")
+            keep.append((fs, arr, o))
+            fptrs.append(arr)
+            outs.append(o)
+        th = A._ptr_array([x[0].h for x in tensors])
+        md = (C.c_int * n_jobs)(*([0] * n_jobs))
+        fp = A._ptr_array([C.cast(a, C.c_void_p).value for a in fptrs])
+        op = A._ptr_array([o.data_ptr() for o in outs])
+        sp = s.cuda_stream
+
+        def single():
+            for (t, _, _), a, o in zip(tensors, fptrs, outs):
+                A.check(L.alto_mttkrp(t.h, 0, R, a, o.data_ptr(), prec, 1, sp), "alto_mttkrp")
+
+        def batched():
+            A.check(L.alto_mttkrp_batched(n_jobs, th, md, R, fp, op, prec, 1, sp), "alto_mttkrp_batched")
+
+        res = {}
+        for fn_name, fn in (("per_job_calls_ms", single), ("batched_call_ms", batched)):
+            for _ in range(3):
+                fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                fn()
+            e1.record(s)
+            torch.cuda.synchronize()
+            res[fn_name] = e0.elapsed_time(e1) / reps
+        res["speedup"] = res["per_job_calls_ms"] / res["batched_call_ms"]
+        out[name] = res
+        log(f"batched MTTKRP {name}: {n_jobs} jobs, per-job calls {res['per_job_calls_ms']:.3f} ms, "
+            f"one batched call {res['batched_call_ms']:.3f} ms (x{res['speedup']:.2f})")
+    del keep, tensors
     torch.cuda.empty_cache()
     return out
 
