@@ -39,4 +39,14 @@ def test_invalid_arguments_rejected_without_gpu_work():
     assert b"invalid" in L.alto_last_error()
     assert L.alto_tensor_destroy(None) == 0
     assert L.alto_mttkrp(None, 0, 16, None, None, 0, 0, None) < 0
+    # multi-tensor MTTKRP: negative count, null tables, null tensor, unknown
+    # precision; an empty batch is a no-op
+    assert L.alto_mttkrp_batched(-1, None, None, 16, None, None, 0, 0, None) < 0
+    assert L.alto_mttkrp_batched(2, None, None, 16, None, None, 0, 0, None) < 0
+    one = (ctypes.c_void_p * 1)(None)
+    modes = (ctypes.c_int * 1)(0)
+    assert L.alto_mttkrp_batched(1, one, modes, 16, one, one, 0, 0, None) < 0
+    assert b"null" in L.alto_last_error()
+    assert L.alto_mttkrp_batched(0, None, None, 16, None, None, 9, 0, None) < 0
+    assert L.alto_mttkrp_batched(0, None, None, 16, None, None, 0, 0, None) == A.ALTO_OK
     assert L.alto_version().startswith(b"alto-mi355x")
