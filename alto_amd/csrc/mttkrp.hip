@@ -594,12 +594,12 @@ int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* mod
     if (njobs == 0) return ALTO_OK;
     if (precision == ALTO_PREC_F64)
         return launch_mttkrp_batch_f64(njobs, tensors, modes, R, (const double* const* const*)factors,
-                                       (const double* const*)vals.data(), (double* const*)outs, k, s);
+                                       (const double* const*)vals.data(), (double* const*)outs, g_seg, k, s);
     if (precision == ALTO_PREC_F32)
         return launch_mttkrp_batch_f32(njobs, tensors, modes, R, (const float* const* const*)factors,
-                                       (const float* const*)vals.data(), (float* const*)outs, k, s);
+                                       (const float* const*)vals.data(), (float* const*)outs, g_seg, k, s);
     return launch_mttkrp_batch_bf16(njobs, tensors, modes, R, (const bf16_t* const* const*)factors,
-                                    (const float* const*)vals.data(), (float* const*)outs, k, s);
+                                    (const float* const*)vals.data(), (float* const*)outs, g_seg, k, s);
 }
 
 int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* factors, double* phi, double eps_div_zero,

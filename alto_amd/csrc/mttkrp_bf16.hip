@@ -10,9 +10,9 @@ int launch_mttkrp_bf16(const alto_tensor* t, int mode, int R, const bf16_t* cons
 }
 
 int launch_mttkrp_batch_bf16(int n, alto_tensor* const* ts, const int* modes, int R, const bf16_t* const* const* U,
-                            const float* const* vals, float* const* outs, const MttkrpTune& k, hipStream_t s)
+                            const float* const* vals, float* const* outs, int seg, const MttkrpTune& k, hipStream_t s)
 {
-    return launch_mttkrp_batch<bf16_t, float, float, float>(n, ts, modes, R, U, vals, outs, k, s);
+    return launch_mttkrp_batch<bf16_t, float, float, float>(n, ts, modes, R, U, vals, outs, seg, k, s);
 }
 
 }  // namespace alto

@@ -10,9 +10,9 @@ int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const*
 }
 
 int launch_mttkrp_batch_f32(int n, alto_tensor* const* ts, const int* modes, int R, const float* const* const* U,
-                            const float* const* vals, float* const* outs, const MttkrpTune& k, hipStream_t s)
+                            const float* const* vals, float* const* outs, int seg, const MttkrpTune& k, hipStream_t s)
 {
-    return launch_mttkrp_batch<float, float, float, float>(n, ts, modes, R, U, vals, outs, k, s);
+    return launch_mttkrp_batch<float, float, float, float>(n, ts, modes, R, U, vals, outs, seg, k, s);
 }
 
 }  // namespace alto

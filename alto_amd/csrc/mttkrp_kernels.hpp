@@ -714,10 +714,11 @@ inline void launch_batch_table(const BatchTable<FT, VT, OT>& tab, int G, int pw,
 // vals[i] into outs[i] (cleared or accumulated by the caller).  Jobs of one
 // lane shape (G in 2..32, E, key width) share launches of up to
 // MTTKRP_BATCH_MAX jobs; the others are launched one by one through
-// launch_mttkrp.  Segment lengths follow the size rule of each tensor.
+// launch_mttkrp.  Segment lengths: `seg` when positive, else the size rule
+// of each tensor.
 template <typename FT, typename VT, typename AT, typename OT>
 int launch_mttkrp_batch(int n, alto_tensor* const* ts, const int* modes, int R, const FT* const* const* U,
-                        const VT* const* vals, OT* const* outs, const MttkrpTune& k, hipStream_t s)
+                        const VT* const* vals, OT* const* outs, int seg, const MttkrpTune& k, hipStream_t s)
 {
     constexpr int E16 = std::is_same<FT, bf16_t>::value ? 4 : 16 / sizeof(FT);
     struct Group {
@@ -727,6 +728,10 @@ int launch_mttkrp_batch(int n, alto_tensor* const* ts, const int* modes, int R, 
     std::vector<Group> groups;
     for (int i = 0; i < n; ++i) {
         const alto_tensor* t = ts[i];
+        if (t->ms[modes[i]].ranked) {  // rank-ordered rows: the hot-row kernels
+            ALTO_TRY((launch_mttkrp<FT, VT, AT, OT>(t, modes[i], R, U[i], vals[i], outs[i], seg, k, s)));
+            continue;
+        }
         StreamArgs<FT, VT, OT> a;
         fill_stream_args(t, modes[i], R, 0, a);
         ALTO_TRY(set_factors(t, modes[i], U[i], a, s));
@@ -734,10 +739,10 @@ int launch_mttkrp_batch(int n, alto_tensor* const* ts, const int* modes, int R, 
         a.out = outs[i];
         int G = 0, E = 0;
         if (!vector_lane_shape(t, modes[i], R, a, k, G, E) || G < 2 || G > 32) {
-            ALTO_TRY((launch_mttkrp<FT, VT, AT, OT>(t, modes[i], R, U[i], vals[i], outs[i], 0, k, s)));
+            ALTO_TRY((launch_mttkrp<FT, VT, AT, OT>(t, modes[i], R, U[i], vals[i], outs[i], seg, k, s)));
             continue;
         }
-        a.seg = (uint64_t)tuned_seg(t->nnz, G, (int)sizeof(AT));
+        a.seg = (uint64_t)(seg > 0 ? seg : tuned_seg(t->nnz, G, (int)sizeof(AT)));
         (void)stream_grid(t->nnz, (int)a.seg, G, a);
         Group* g = nullptr;
         for (auto& x : groups)
@@ -781,10 +786,10 @@ int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const*
 int launch_mttkrp_bf16(const alto_tensor* t, int mode, int R, const bf16_t* const* U, const float* val, float* out,
                        int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_batch_f64(int n, alto_tensor* const* ts, const int* modes, int R, const double* const* const* U,
-                            const double* const* vals, double* const* outs, const MttkrpTune& k, hipStream_t s);
+                            const double* const* vals, double* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_batch_f32(int n, alto_tensor* const* ts, const int* modes, int R, const float* const* const* U,
-                            const float* const* vals, float* const* outs, const MttkrpTune& k, hipStream_t s);
+                            const float* const* vals, float* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_batch_bf16(int n, alto_tensor* const* ts, const int* modes, int R, const bf16_t* const* const* U,
-                             const float* const* vals, float* const* outs, const MttkrpTune& k, hipStream_t s);
+                            const float* const* vals, float* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
 
 }  // namespace alto

@@ -113,9 +113,9 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
  * accumulate): factors[i] holds that tensor's nmodes device pointers.  Jobs
  * that take the vector-lane kernel with the same lanes per nonzero, elements
  * per lane and key width share a launch of up to 16 jobs; each job keeps its
- * own segment length and grid, laid end to end with the others'.  Other jobs
- * are launched one by one.  Shapes chosen by alto_mttkrp_autotune are not
- * used here.  Each output equals that job's alto_mttkrp up to the order of
+ * own segment length (ALTO_TUNE_SEG when set, else the size rule) and grid,
+ * laid end to end with the others'.  Other jobs are launched one by one.
+ * Shapes chosen by alto_mttkrp_autotune are not used here.  Each output equals that job's alto_mttkrp up to the order of
  * the floating-point atomic additions.  Asynchronous on `stream`. */
 int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* modes, uint64_t rank,
                         const void* const* const* factors, void* const* outs, int precision, int accumulate,
