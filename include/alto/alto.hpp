@@ -8,6 +8,7 @@
 //   AltoTensor<LIT>   (include/alto.hpp:27)     same scalar fields + device handle
 //   create_alto       (include/alto.hpp:108)    device linearize / sort / partition
 //   mttkrp_alto[_par] (include/alto.hpp:457,617) device MTTKRP, adds into factors[mode]
+//   mttkrp_alto_batched (none)                  many tensors' MTTKRPs in one launch
 //   cpd_alto          (include/cpd.hpp:43)      device CP-ALS, updates M in place
 //   cp_apr_alto       (include/apr.hpp:984)     device CP-APR, updates M in place
 //
@@ -21,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <memory>
 #include <vector>
 
 #include "../alto_hip.h"
@@ -91,7 +93,8 @@ template <typename LIT> inline int lit_bits() { return (int)(8 * sizeof(LIT)); }
 template <typename LIT> inline LIT lit_from(const uint64_t* w, int words)
 {
     LIT v = (LIT)w[0];
-    if (words == 2 && sizeof(LIT) == 16) v |= ((LIT)w[1]) << 64;
+    if constexpr (sizeof(LIT) == 16)
+        if (words == 2) v |= ((LIT)w[1]) << 64;
     return v;
 }
 
@@ -198,6 +201,35 @@ template <typename LIT>
 void mttkrp_alto(int target_mode, FType** factors, IType rank, AltoTensor<LIT>* at)
 {
     mttkrp_alto_par(target_mode, factors, rank, at, nullptr, nullptr);
+}
+
+// Many MTTKRPs in one device launch (no reference counterpart; there each
+// tensor takes its own mttkrp_alto_par call): for every job i,
+// factors[i][target_modes[i]] += the MTTKRP of ats[i] along target_modes[i]
+// with factors[i]; one rank for all jobs.
+template <typename LIT>
+static inline void mttkrp_alto_batched(int njobs, const int* target_modes, FType*** factors, IType rank,
+                                       AltoTensor<LIT>** ats)
+{
+    using namespace alto_dropin;
+    std::vector<std::unique_ptr<DeviceFactors>> d;
+    std::vector<alto_tensor_t*> ts(njobs);
+    std::vector<const void* const*> fp(njobs);
+    std::vector<void*> outs(njobs);
+    for (int i = 0; i < njobs; ++i) {
+        d.emplace_back(new DeviceFactors(ats[i]->nmode, ats[i]->dims, rank, factors[i]));
+        ts[i] = ats[i]->dev;
+        fp[i] = (const void* const*)d[i]->p.data();
+        outs[i] = d[i]->p[target_modes[i]];
+    }
+    check(alto_mttkrp_batched(njobs, ts.data(), target_modes, rank, fp.data(), outs.data(), ALTO_PREC_F64, 1,
+                              nullptr),
+          "mttkrp_alto_batched");
+    check(alto_device_synchronize(), "synchronize");
+    for (int i = 0; i < njobs; ++i) {
+        const int m = target_modes[i];
+        check(alto_memcpy_d2h(factors[i][m], d[i]->p[m], ats[i]->dims[m] * rank * sizeof(FType)), "alto_memcpy_d2h");
+    }
 }
 
 // ---------------------------------------------------------------- Kruskal model

@@ -23,7 +23,9 @@ int main() {
     KruskalModel* M; CreateKruskalModel(3, dims, 2, &M);
     AltoTensor<unsigned long long>* A64 = nullptr; AltoTensor<unsigned __int128>* A128 = nullptr;
     if (A64) { cpd_alto(A64, M, 1, 1e-5); cp_apr_alto(A128, M, 1, 1, 0.0, 0.0, 1e-10, 1e-2, true);
-               FType** of; create_da_mem(-1, 2, A64, &of); mttkrp_alto_par(0, M->U, 2, A64, nullptr, of); }
+               FType** of; create_da_mem(-1, 2, A64, &of); mttkrp_alto_par(0, M->U, 2, A64, nullptr, of);
+               int modes[1] = {0}; FType** fs[1] = {M->U}; AltoTensor<unsigned long long>* ts[1] = {A64};
+               mttkrp_alto_batched(1, modes, fs, 2, ts); }
     DestroyKruskalModel(M);
     return 0;
 }
@@ -224,3 +226,86 @@ def test_poisson_generator_matches_oracle(tmp_path, dims, edges, rank, threads):
     assert all((cidx[m] < dims[m]).all() for m in range(len(dims)))
     key = [tuple(int(cidx[m, i]) for m in reversed(range(len(dims)))) for i in range(cidx.shape[1])]
     assert all(a < b for a, b in zip(key, key[1:]))
+
+
+_BATCH_PROG = r'''
+#include <math.h>
+#include "alto/alto.hpp"
+// three tensors of 3-4 modes (LCG coordinates, duplicates allowed), rank 8:
+// mttkrp_alto_batched equals mttkrp_alto_par per tensor and brute force
+static unsigned long long s = 12345;
+static unsigned long long nxt() { s = s * 6364136223846793005ull + 1442695040888963407ull; return s >> 33; }
+int main() {
+    const int J = 3, R = 8;
+    IType dimsv[J][4] = {{30, 40, 50, 0}, {7, 300, 20, 9}, {64, 64, 64, 0}};
+    int nm[J] = {3, 4, 3}, modes[J] = {0, 2, 1};
+    IType nnzv[J] = {4000, 3000, 20000};
+    SparseTensor spt[J];
+    AltoTensor<unsigned long long>* at[J];
+    FType** A[J]; FType** B[J];
+    for (int j = 0; j < J; ++j) {
+        spt[j].nmodes = nm[j]; spt[j].dims = dimsv[j]; spt[j].nnz = nnzv[j];
+        spt[j].cidx = new IType*[nm[j]]; spt[j].vals = new FType[nnzv[j]];
+        for (int m = 0; m < nm[j]; ++m) {
+            spt[j].cidx[m] = new IType[nnzv[j]];
+            for (IType i = 0; i < nnzv[j]; ++i) spt[j].cidx[m][i] = nxt() % dimsv[j][m];
+        }
+        for (IType i = 0; i < nnzv[j]; ++i) spt[j].vals[i] = 0.5 + (nxt() % 1000) / 1000.0;
+        create_alto(&spt[j], &at[j], 4);
+        A[j] = new FType*[nm[j]]; B[j] = new FType*[nm[j]];
+        for (int m = 0; m < nm[j]; ++m) {
+            A[j][m] = new FType[dimsv[j][m] * R]; B[j][m] = new FType[dimsv[j][m] * R];
+            for (IType k = 0; k < dimsv[j][m] * R; ++k) A[j][m][k] = B[j][m][k] = (nxt() % 1000) / 1000.0;
+        }
+    }
+    FType* C0[J];  // the target factors before the calls
+    for (int j = 0; j < J; ++j) {
+        const IType n = dimsv[j][modes[j]] * R;
+        C0[j] = new FType[n];
+        for (IType k = 0; k < n; ++k) C0[j][k] = A[j][modes[j]][k];
+    }
+    for (int j = 0; j < J; ++j) mttkrp_alto_par(modes[j], A[j], R, at[j], nullptr, nullptr);
+    mttkrp_alto_batched(J, modes, B, R, at);
+    double worst = 0;
+    for (int j = 0; j < J; ++j) {
+        const int t = modes[j];
+        const IType I = dimsv[j][t];
+        // brute force over the coordinates, added to the initial values
+        double* bf = new double[I * R]();
+        for (IType i = 0; i < nnzv[j]; ++i)
+            for (int r = 0; r < R; ++r) {
+                double p = spt[j].vals[i];
+                for (int m = 0; m < nm[j]; ++m)
+                    if (m != t) p *= A[j][m][spt[j].cidx[m][i] * R + r];
+                bf[spt[j].cidx[t][i] * R + r] += p;
+            }
+        double mx = 0;
+        for (IType k = 0; k < I * R; ++k) mx = fmax(mx, fabs(bf[k]));
+        for (IType k = 0; k < I * R; ++k) {
+            worst = fmax(worst, fabs(A[j][t][k] - B[j][t][k]) / mx);
+            worst = fmax(worst, fabs(B[j][t][k] - C0[j][k] - bf[k]) / mx);
+        }
+        delete[] bf;
+    }
+    printf("max rel diff batched vs per-tensor %.3e\n", worst);
+    for (int j = 0; j < J; ++j) destroy_alto(at[j]);
+    return worst < 1e-12 ? 0 : 1;
+}
+'''
+
+
+@pytest.mark.gpu
+def test_dropin_batched_mttkrp_matches_per_tensor(tmp_path):
+    """The C++ drop-in mttkrp_alto_batched over three tensors gives what
+    mttkrp_alto_par gives per tensor and what brute force gives (both add
+    into the target factor)."""
+    src = tmp_path / "batch.cpp"
+    src.write_text(_BATCH_PROG)
+    exe = tmp_path / "batch"
+    libdir = os.path.join(ROOT, "alto_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L" + libdir, "-lalto_hip", "-Wl,-rpath," + libdir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "max rel diff" in r.stdout
