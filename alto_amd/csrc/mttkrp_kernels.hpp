@@ -705,14 +705,15 @@ inline void launch_batch_table(const BatchTable<FT, VT, OT>& tab, int G, int pw,
     case 4: KB_(4); break;
     case 8: KB_(8); break;
     case 16: KB_(16); break;
-    default: KB_(32); break;
+    case 32: KB_(32); break;
+    default: KB_(64); break;
     }
 #undef KB_
 }
 
 // Job i: the MTTKRP of ts[i] along modes[i] with factors U[i] and values
 // vals[i] into outs[i] (cleared or accumulated by the caller).  Jobs of one
-// lane shape (G in 2..32, E, key width) share launches of up to
+// lane shape (G in 2..64, E, key width) share launches of up to
 // MTTKRP_BATCH_MAX jobs; the others are launched one by one through
 // launch_mttkrp.  Segment lengths: `seg` when positive, else the size rule
 // of each tensor.
@@ -738,7 +739,7 @@ int launch_mttkrp_batch(int n, alto_tensor* const* ts, const int* modes, int R, 
         a.val = vals[i];
         a.out = outs[i];
         int G = 0, E = 0;
-        if (!vector_lane_shape(t, modes[i], R, a, k, G, E) || G < 2 || G > 32) {
+        if (!vector_lane_shape(t, modes[i], R, a, k, G, E) || G < 2 || G > 64) {
             ALTO_TRY((launch_mttkrp<FT, VT, AT, OT>(t, modes[i], R, U[i], vals[i], outs[i], seg, k, s)));
             continue;
         }
