@@ -94,6 +94,7 @@ def lib():
         "alto_cpd_session_create_seeded": (i32, [vp, u64, i32, u64, vp, C.POINTER(vp)]),
         "alto_cpd_session_iterate": (i32, [vp, i32, f64, vp, C.POINTER(i32), C.POINTER(i32), i32]),
         "alto_cpd_session_get": (i32, [vp, vp, vp, C.POINTER(f64), C.POINTER(CpdTimers)]),
+        "alto_cpd_sessions_iterate": (i32, [i32, vp, i32, f64, vp, vp]),
         "alto_cpd_session_destroy": (i32, [vp]),
         "alto_pseudo_inverse": (i32, [vp, i32, i32, u64, vp, u64, C.POINTER(i32), vp]),
         "alto_tune_set": (i32, [i32, i32]),
@@ -367,6 +368,16 @@ class CpdSession:
         if h and _lib is not None:
             _lib.alto_cpd_session_destroy(h)
             self.h = None
+
+
+def cpd_sessions_iterate(sessions, n, epsilon=0.0, stream=None):
+    """alto_cpd_sessions_iterate: the sessions (one rank and precision) run up
+    to n CP-ALS iterations in lockstep; returns the iterations run per session."""
+    k = len(sessions)
+    ran = (C.c_int * max(k, 1))()
+    check(lib().alto_cpd_sessions_iterate(k, _ptr_array([s.h for s in sessions]), n, epsilon, stream,
+                                          C.cast(ran, C.c_void_p)), "cpd_sessions_iterate")
+    return [ran[i] for i in range(k)]
 
 
 def _factor_arrays(factors, precision):

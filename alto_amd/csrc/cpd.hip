@@ -176,6 +176,9 @@ static int cpd_download(alto_tensor* t, CpdState<UT, ST, MT>& c, double* const* 
     return ALTO_OK;
 }
 
+template <typename UT, typename ST, typename MT>
+static int cpd_mode_rest(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it, PhaseTimer& pt, hipStream_t s);
+
 // one mode update (cpd_alto's loop body, include/cpd.hpp:103-138)
 template <typename UT, typename ST, typename MT>
 static int cpd_mode_step(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it, PhaseTimer& pt, hipStream_t s)
@@ -187,6 +190,15 @@ static int cpd_mode_step(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it,
     // M is zero here: cleared at set-up and by the previous mode's last launch
     ALTO_TRY(mttkrp_device(t, j, R, fptr, c.M.p, c.prec, 1, s, 0));
     ALTO_TRY(pt.mark(PH_MTTKRP));
+    return cpd_mode_rest(t, c, j, it, pt, s);
+}
+
+// the mode update after its MTTKRP (M holds M_j): R x R solve, row solves,
+// λ, Gram + scaling (which also clears M for the next mode)
+template <typename UT, typename ST, typename MT>
+static int cpd_mode_rest(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it, PhaseTimer& pt, hipStream_t s)
+{
+    const int R = c.R, N = c.N;
     ALTO_TRY(chol_device(c.grams.template as<double>(), N, j, R, c.L.template as<double>(), c.P.template as<double>(),
                          c.wsA.template as<double>(), c.wsQ.template as<double>(), c.status.template as<int>(), s));
     // the last mode's solve also leaves the fit's inner-product partials
@@ -235,6 +247,13 @@ struct CpdSessionBase {
     virtual ~CpdSessionBase() {}
     virtual int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) = 0;
     virtual int get(double* const* Uh, double* lambda_h) = 0;
+    // pieces of one eager iteration, for iterating several sessions in
+    // lockstep on one stream: the storage factors and M of mode j, the rest
+    // of mode j after its MTTKRP, the fit (left in the session's fit buffer)
+    virtual int nmodes() const = 0;
+    virtual void mode_io(int j, const void** fptr, void** M) = 0;
+    virtual int mode_rest(int j, hipStream_t st) = 0;
+    virtual int fit_launch(hipStream_t st, double* fit_dst) = 0;
 };
 
 int graphs_enabled();
@@ -309,6 +328,24 @@ struct CpdSession : CpdSessionBase {
         ALTO_TRY(normsq_device(t->vals.as<double>(), t->nnz, c.nxpart.template as<double>(), c.nx.template as<double>(), s));
         HIP_TRY(hipMemcpyAsync(&normX, c.nx.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        return ALTO_OK;
+    }
+    int nmodes() const override { return c.N; }
+    void mode_io(int j, const void** fptr, void** M) override
+    {
+        for (int m = 0; m < c.N; ++m) fptr[m] = c.st(m);
+        *M = c.M.p;
+    }
+    int mode_rest(int j, hipStream_t st) override
+    {
+        PhaseTimer pt;
+        ALTO_TRY(pt.init(false, st));
+        return cpd_mode_rest(t, c, j, it, pt, st);
+    }
+    int fit_launch(hipStream_t st, double* fit_dst) override
+    {
+        ALTO_TRY(cpd_fit_step(t, c, normX, st));
+        HIP_TRY(hipMemcpyAsync(fit_dst, c.fitbuf.p, sizeof(double), hipMemcpyDeviceToHost, st));
         return ALTO_OK;
     }
     int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) override
@@ -545,6 +582,92 @@ int alto_cpd_session_iterate(alto_cpd_session_t* sess, int max_iters, double eps
     CpdSessionBase* b = reinterpret_cast<CpdSessionBase*>(sess);
     ALTO_TRY(b->iterate(max_iters, epsilon, fit_trace, iters, verbose));
     if (converged) *converged = b->converged ? 1 : 0;
+    return ALTO_OK;
+}
+
+int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int max_iters, double epsilon,
+                              alto_stream_t stream, int* iters)
+{
+    if (n < 0 || max_iters < 0 || (n > 0 && !sessions)) {
+        set_error("alto_cpd_sessions_iterate: invalid argument");
+        return ALTO_ERR_ARG;
+    }
+    std::vector<CpdSessionBase*> b(n);
+    for (int i = 0; i < n; ++i) {
+        b[i] = reinterpret_cast<CpdSessionBase*>(sessions[i]);
+        if (!b[i]) {
+            set_error("alto_cpd_sessions_iterate: null session %d", i);
+            return ALTO_ERR_ARG;
+        }
+        if (b[i]->R != b[0]->R || b[i]->prec != b[0]->prec) {
+            set_error("alto_cpd_sessions_iterate: session %d differs in rank or precision from session 0", i);
+            return ALTO_ERR_ARG;
+        }
+        for (int k = 0; k < i; ++k)
+            if (b[k] == b[i]) {
+                set_error("alto_cpd_sessions_iterate: session %d is listed twice", i);
+                return ALTO_ERR_ARG;
+            }
+        if (iters) iters[i] = 0;
+    }
+    if (n == 0) return ALTO_OK;
+    hipStream_t s = (hipStream_t)stream;
+    for (auto* x : b) HIP_TRY(hipStreamSynchronize(x->s));  // earlier work of each session
+    double* fits = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&fits, sizeof(double) * n, hipHostMallocDefault));
+    struct FitsGuard {
+        double* p;
+        ~FitsGuard() { (void)hipHostFree(p); }
+    } guard{fits};
+    std::vector<alto_tensor_t*> ts;
+    std::vector<int> modes;
+    std::vector<std::vector<const void*>> fp(n, std::vector<const void*>(ALTO_MAXN));
+    std::vector<const void* const*> fpp;
+    std::vector<void*> outs;
+    std::vector<int> act;
+    for (int k = 0; k < max_iters; ++k) {
+        act.clear();
+        int maxN = 0;
+        for (int i = 0; i < n; ++i)
+            if (!b[i]->converged) {
+                act.push_back(i);
+                maxN = std::max(maxN, b[i]->nmodes());
+            }
+        if (act.empty()) break;
+        const double t0 = now_s();
+        for (int j = 0; j < maxN; ++j) {
+            ts.clear();
+            modes.clear();
+            fpp.clear();
+            outs.clear();
+            for (int i : act)
+                if (j < b[i]->nmodes()) {
+                    void* M = nullptr;
+                    b[i]->mode_io(j, fp[i].data(), &M);
+                    ts.push_back(b[i]->t);
+                    modes.push_back(j);
+                    fpp.push_back(fp[i].data());
+                    outs.push_back(M);
+                }
+            // M is zero here (cleared at set-up and by each mode's last launch)
+            ALTO_TRY(alto_mttkrp_batched((int)ts.size(), ts.data(), modes.data(), (uint64_t)b[0]->R, fpp.data(),
+                                         outs.data(), b[0]->prec, 1, stream));
+            for (int i : act)
+                if (j < b[i]->nmodes()) ALTO_TRY(b[i]->mode_rest(j, s));
+        }
+        for (int i : act) ALTO_TRY(b[i]->fit_launch(s, fits + i));
+        HIP_TRY(hipStreamSynchronize(s));
+        const double dt = now_s() - t0;
+        for (int i : act) {
+            CpdSessionBase* x = b[i];
+            x->fit = fits[i];
+            if (x->it > 0 && fabs(x->prev_fit - x->fit) < epsilon) x->converged = true;
+            x->prev_fit = x->fit;
+            ++x->it;
+            x->total += dt;
+            if (iters) ++iters[i];
+        }
+    }
     return ALTO_OK;
 }
 

@@ -210,6 +210,18 @@ int alto_cpd_session_create_seeded(alto_tensor_t* t, uint64_t rank, int precisio
                                    alto_stream_t stream, alto_cpd_session_t** out);
 int alto_cpd_session_iterate(alto_cpd_session_t* s, int max_iters, double epsilon, double* fit_trace,
                              int* iters, int* converged, int verbose);
+/* Several sessions iterated in lockstep on one stream (no reference
+ * counterpart: the reference runs one cpd_alto per process).  Each
+ * iteration runs, for every mode index j, one alto_mttkrp_batched over the
+ * sessions that have a mode j and have not converged, then each of their
+ * mode-j solves, then all their fits, read with one synchronization.  Every
+ * session ends where alto_cpd_session_iterate(max_iters, epsilon) would leave
+ * it, up to the order of the MTTKRP's floating-point atomic additions.  All
+ * sessions share the rank and the precision; earlier work on each session's
+ * own stream is waited for; the sessions' graphs and phase timers are not
+ * used.  iters (may be null): iterations run per session. */
+int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int max_iters, double epsilon,
+                              alto_stream_t stream, int* iters);
 int alto_cpd_session_get(alto_cpd_session_t* s, double* const* U, double* lambda, double* fit,
                          alto_cpd_timers_t* timers);
 int alto_cpd_session_destroy(alto_cpd_session_t* s);

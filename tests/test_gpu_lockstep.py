@@ -1,0 +1,95 @@
+"""GPU parity of alto_cpd_sessions_iterate: sessions of uneven tensors
+iterated in lockstep (one batched MTTKRP per mode index, then each session's
+solves, then all fits) end where each session's own iterate leaves it --
+FP64 within 1e-10, FP32 within 1e-5 and BF16 within 1e-3 of the fit -- and
+stop at the same iteration under a fit tolerance."""
+import numpy as np
+import pytest
+
+from tests._tensors import uniform_tensor, zipf_tensor
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [
+    ([60, 70, 80], 8000, "uniform"),
+    ([300, 200, 500], 40000, "zipf"),
+    ([30, 20, 10, 40], 5000, "uniform"),
+    ([40, 30, 20, 25, 15], 6000, "uniform"),
+    ([500, 3000, 40], 20000, "zipf"),
+]
+
+
+def _A():
+    import alto_amd
+    return alto_amd
+
+
+def _tensors(specs, seed0=0):
+    A = _A()
+    out = []
+    for k, (dims, nnz, kind) in enumerate(specs):
+        gen = zipf_tensor if kind == "zipf" else uniform_tensor
+        cidx, vals = gen(dims, nnz, 200 + seed0 + k)
+        out.append((dims, A.AltoTensor(dims, cidx, vals, nprtn=8)))
+    return out
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5), (2, 1e-3)])
+def test_lockstep_equals_each_session_alone(prec, tol):
+    A = _A()
+    ts = _tensors(SPECS)
+    R, iters = 16, 8
+    alone, lock = [], []
+    for k, (dims, g) in enumerate(ts):
+        U, _ = A.kruskal_random_init(dims, R, 10 + k)
+        alone.append(A.CpdSession(g, U, precision=prec))
+        lock.append(A.CpdSession(g, U, precision=prec))
+    for s in alone:
+        s.iterate(iters)
+    ran = A.cpd_sessions_iterate(lock, iters)
+    assert ran == [iters] * len(ts)
+    for (dims, _), a, b in zip(ts, alone, lock):
+        Ua, la, fa, _ = a.get()
+        Ub, lb, fb, _ = b.get()
+        assert abs(fa - fb) <= tol, (dims, fa, fb)
+        if prec == 0:
+            np.testing.assert_allclose(lb, la, rtol=1e-8)
+            for ua, ub in zip(Ua, Ub):
+                np.testing.assert_allclose(ub, ua, rtol=1e-7, atol=1e-9)
+
+
+def test_lockstep_convergence_and_continuation():
+    """Under a fit tolerance each session stops where it would alone, the
+    others go on; a later lockstep call continues the iteration count."""
+    A = _A()
+    ts = _tensors(SPECS[:4], seed0=5)
+    R = 8
+    alone, lock = [], []
+    for k, (dims, g) in enumerate(ts):
+        U, _ = A.kruskal_random_init(dims, R, 30 + k)
+        alone.append(A.CpdSession(g, U))
+        lock.append(A.CpdSession(g, U))
+    want = []
+    for s in alone:
+        f, _ = s.iterate(60, 1e-4)
+        want.append(len(f))
+    ran = A.cpd_sessions_iterate(lock, 3, 1e-4)
+    ran2 = A.cpd_sessions_iterate(lock, 57, 1e-4)
+    assert [a + b for a, b in zip(ran, ran2)] == want
+    for a, b in zip(alone, lock):
+        assert abs(a.get()[2] - b.get()[2]) <= 1e-10
+
+
+def test_lockstep_arguments():
+    A = _A()
+    ts = _tensors(SPECS[:2])
+    U0, _ = A.kruskal_random_init(ts[0][0], 8, 1)
+    U1, _ = A.kruskal_random_init(ts[1][0], 16, 1)
+    s0 = A.CpdSession(ts[0][1], U0)
+    s1 = A.CpdSession(ts[1][1], U1)
+    assert A.cpd_sessions_iterate([], 5) == []
+    with pytest.raises(A.AltoError):  # ranks differ
+        A.cpd_sessions_iterate([s0, s1], 2)
+    with pytest.raises(A.AltoError):  # one session twice
+        A.cpd_sessions_iterate([s0, s0], 2)
+    assert A.cpd_sessions_iterate([s0], 0) == [0]
