@@ -400,7 +400,9 @@ def batched_leg(n_jobs=16, reps=50):
     alto_mttkrp call per job, on one stream, timed with events on that stream
     over `reps` back-to-back calls (host submission included).  16 tensors of
     3-5 modes, dims log-uniform in [64, 2^16], nnz log-uniform in [1e4, 2e5],
-    Zipf(1.1), lognormal values; mode-0 MTTKRP accumulated into the output."""
+    Zipf(1.1), lognormal values; mode-0 MTTKRP accumulated into the output.
+    Then CP-ALS on the same tensors, rank 16: each session one after another
+    against all in lockstep, wall clock."""
     import ctypes as C
 
     import torch
@@ -462,7 +464,29 @@ def batched_leg(n_jobs=16, reps=50):
         out[name] = res
         log(f"batched MTTKRP {name}: {n_jobs} jobs, per-job calls {res['per_job_calls_ms']:.3f} ms, "
             f"one batched call {res['batched_call_ms']:.3f} ms (x{res['speedup']:.2f})")
-    del keep, tensors
+    del keep
+    # CP-ALS on the same tensors: 20 iterations of each session one after
+    # another against all sessions in lockstep (alto_cpd_sessions_iterate)
+    for R, prec, name in ((16, A.PREC_F64, "cpals_r16_fp64"), (16, A.PREC_F32, "cpals_r16_fp32")):
+        inits = [A.kruskal_random_init(dims, R, 50 + k)[0] for k, (_, dims, _) in enumerate(tensors)]
+        alone = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
+        lock = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for x in alone:
+            x.iterate(20)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        A.cpd_sessions_iterate(lock, 20)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        dfit = max(abs(x.get()[2] - y.get()[2]) for x, y in zip(alone, lock))
+        out[name] = {"iterations": 20, "one_after_another_ms": (t1 - t0) * 1e3, "lockstep_ms": (t2 - t1) * 1e3,
+                     "speedup": (t1 - t0) / (t2 - t1), "max_fit_delta": dfit}
+        log(f"lockstep CP-ALS {name}: {n_jobs} sessions x 20 iterations, one after another {(t1 - t0) * 1e3:.1f} ms, "
+            f"lockstep {(t2 - t1) * 1e3:.1f} ms (x{(t1 - t0) / (t2 - t1):.2f}), max fit delta {dfit:.1e}")
+        del alone, lock
+    del tensors
     torch.cuda.empty_cache()
     return out
 
