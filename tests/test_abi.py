@@ -49,4 +49,10 @@ def test_invalid_arguments_rejected_without_gpu_work():
     assert b"null" in L.alto_last_error()
     assert L.alto_mttkrp_batched(0, None, None, 16, None, None, 9, 0, None) < 0
     assert L.alto_mttkrp_batched(0, None, None, 16, None, None, 0, 0, None) == A.ALTO_OK
+    # lockstep sessions: negative counts, null table, null session; none is a no-op
+    assert L.alto_cpd_sessions_iterate(-1, None, 1, 0.0, None, None) < 0
+    assert L.alto_cpd_sessions_iterate(1, None, 1, 0.0, None, None) < 0
+    assert L.alto_cpd_sessions_iterate(1, one, 1, 0.0, None, None) < 0
+    assert L.alto_cpd_sessions_iterate(1, one, -1, 0.0, None, None) < 0
+    assert L.alto_cpd_sessions_iterate(0, None, 5, 0.0, None, None) == A.ALTO_OK
     assert L.alto_version().startswith(b"alto-mi355x")
