@@ -619,6 +619,43 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
         double* p;
         ~FitsGuard() { (void)hipHostFree(p); }
     } guard{fits};
+    // the sessions' solve chains are independent: they run on NAUX private
+    // streams, forked from `s` after each batched MTTKRP and joined back
+    // before the next one (the next MTTKRP accumulates into the M that each
+    // chain's last launch clears)
+    constexpr int NAUX = 4;
+    struct Aux {
+        hipStream_t st[NAUX] = {};
+        hipEvent_t fork = nullptr, join[NAUX] = {};
+        ~Aux()
+        {
+            for (auto x : st)
+                if (x) (void)hipStreamDestroy(x);
+            for (auto e : join)
+                if (e) (void)hipEventDestroy(e);
+            if (fork) (void)hipEventDestroy(fork);
+        }
+    } aux;
+    const int naux = n > 1 ? NAUX : 0;
+    for (int k = 0; k < naux; ++k) {
+        HIP_TRY(hipStreamCreateWithFlags(&aux.st[k], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&aux.join[k], hipEventDisableTiming));
+    }
+    if (naux) HIP_TRY(hipEventCreateWithFlags(&aux.fork, hipEventDisableTiming));
+    auto fork = [&]() -> int {
+        if (!naux) return ALTO_OK;
+        HIP_TRY(hipEventRecord(aux.fork, s));
+        for (int k = 0; k < naux; ++k) HIP_TRY(hipStreamWaitEvent(aux.st[k], aux.fork, 0));
+        return ALTO_OK;
+    };
+    auto join = [&]() -> int {
+        for (int k = 0; k < naux; ++k) {
+            HIP_TRY(hipEventRecord(aux.join[k], aux.st[k]));
+            HIP_TRY(hipStreamWaitEvent(s, aux.join[k], 0));
+        }
+        return ALTO_OK;
+    };
+    auto lane = [&](int q) { return naux ? aux.st[q % naux] : s; };
     std::vector<alto_tensor_t*> ts;
     std::vector<int> modes;
     std::vector<std::vector<const void*>> fp(n, std::vector<const void*>(ALTO_MAXN));
@@ -652,10 +689,16 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
             // M is zero here (cleared at set-up and by each mode's last launch)
             ALTO_TRY(alto_mttkrp_batched((int)ts.size(), ts.data(), modes.data(), (uint64_t)b[0]->R, fpp.data(),
                                          outs.data(), b[0]->prec, 1, stream));
+            ALTO_TRY(fork());
+            int q = 0;
             for (int i : act)
-                if (j < b[i]->nmodes()) ALTO_TRY(b[i]->mode_rest(j, s));
+                if (j < b[i]->nmodes()) ALTO_TRY(b[i]->mode_rest(j, lane(q++)));
+            ALTO_TRY(join());
         }
-        for (int i : act) ALTO_TRY(b[i]->fit_launch(s, fits + i));
+        ALTO_TRY(fork());
+        int q = 0;
+        for (int i : act) ALTO_TRY(b[i]->fit_launch(lane(q++), fits + i));
+        ALTO_TRY(join());
         HIP_TRY(hipStreamSynchronize(s));
         const double dt = now_s() - t0;
         for (int i : act) {

@@ -214,7 +214,9 @@ int alto_cpd_session_iterate(alto_cpd_session_t* s, int max_iters, double epsilo
  * counterpart: the reference runs one cpd_alto per process).  Each
  * iteration runs, for every mode index j, one alto_mttkrp_batched over the
  * sessions that have a mode j and have not converged, then each of their
- * mode-j solves, then all their fits, read with one synchronization.  Every
+ * mode-j solves (spread over 4 internal streams, joined back to `stream`
+ * before the next MTTKRP), then all their fits, read with one
+ * synchronization.  Every
  * session ends where alto_cpd_session_iterate(max_iters, epsilon) would leave
  * it, up to the order of the MTTKRP's floating-point atomic additions.  All
  * sessions share the rank and the precision; earlier work on each session's
