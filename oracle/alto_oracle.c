@@ -982,9 +982,22 @@ double or_loglikelihood(or_alto *at, double **U, double *lambda, uint64_t R, dou
     return logsum - fsum;
 }
 
-/* cp_apr_alto, include/apr.hpp:984-1376 */
-int or_cp_apr(or_alto *at, uint64_t R, int max_iters, int max_inner, double eps, double eps_div_zero,
-              double kappa_tol, double kappa, int dobench, double **U, double *lambda, or_apr_result *res)
+/* Planted mistakes, compiled only into the mutant libraries the oracle's own
+ * tests build (OR_MUTANT = 1..4) to show that those tests catch them. */
+#ifndef OR_MUTANT
+#define OR_MUTANT 0
+#endif
+#if OR_MUTANT == 1
+#define OR_KKT_PICK fmax /* instead of fmin */
+#else
+#define OR_KKT_PICK fmin
+#endif
+
+/* cp_apr_alto, include/apr.hpp:984-1376, with an optional observer called at
+ * the driver's steps (test infrastructure: the observer only reads). */
+int or_cp_apr_hooked(or_alto *at, uint64_t R, int max_iters, int max_inner, double eps, double eps_div_zero,
+                     double kappa_tol, double kappa, int dobench, double **U, double *lambda, or_apr_result *res,
+                     or_apr_hook hook, void *ctx)
 {
     int N = at->nmode;
     double *Phi[OR_MAX_MODES];
@@ -993,35 +1006,66 @@ int or_cp_apr(or_alto *at, uint64_t R, int max_iters, int max_inner, double eps,
     double *kkt = (double *)calloc(max_iters + 1, sizeof(double));
     int *n_inner = (int *)calloc(max_iters + 1, sizeof(int));
     memset(res, 0, sizeof(*res));
+    /* KruskalModelNormalize, include/apr.hpp:1105 */
     or_normalize_all_l1(N, at->dims, R, U, lambda);
+    if (hook) hook(ctx, OR_APR_EV_START, 0, -1, -1, U, lambda, NULL, 0.0);
     int iter;
     for (iter = 0; iter < max_iters; ++iter) {
         int converged = 0;
         for (int n = 0; n < N; ++n) {
             uint64_t dim = at->dims[n];
             double *B = U[n], *Ph = Phi[n];
-            if (iter > 0)
+            /* kappa fix, include/apr.hpp:1176-1189 */
+            if (iter > 0) {
+                if (hook) hook(ctx, OR_APR_EV_PRE_FIX, iter, n, -1, U, lambda, Ph, 0.0);
                 for (uint64_t k = 0; k < dim * R; ++k)
+#if OR_MUTANT == 2
+                    if (B[k] < kappa_tol && Ph[k] < 1.0) B[k] += kappa;
+#else
                     if (B[k] < kappa_tol && Ph[k] > 1.0) B[k] += kappa;
-            or_redistribute_lambda(dim, R, B, lambda);
+#endif
+                if (hook) hook(ctx, OR_APR_EV_POST_FIX, iter, n, -1, U, lambda, Ph, 0.0);
+            }
+            or_redistribute_lambda(dim, R, B, lambda); /* :1192 */
             for (int inner = 0; inner < max_inner; ++inner) {
                 n_inner[iter]++;
                 res->inner_per_mode[n]++;
-                or_phi(at, n, U, R, eps_div_zero, Ph);
+                or_phi(at, n, U, R, eps_div_zero, Ph); /* :1205 */
+                /* KKT violation, :1207-1215 */
                 double kv = 0.0;
                 for (uint64_t k = 0; k < dim * R; ++k) {
-                    double mc = fabs(fmin(B[k], 1.0 - Ph[k]));
+                    double mc = fabs(OR_KKT_PICK(B[k], 1.0 - Ph[k]));
                     kv = fmax(kv, mc);
                 }
                 kkt_mode[n] = kv;
-                if (kv < eps && !dobench) break;
-                for (uint64_t k = 0; k < dim * R; ++k) B[k] *= Ph[k];
+                if (hook) hook(ctx, OR_APR_EV_PHI, iter, n, inner, U, lambda, Ph, kv);
+                if (kv < eps && !dobench) break; /* :1217-1221 */
+                for (uint64_t k = 0; k < dim * R; ++k) B[k] *= Ph[k]; /* multiplicative update, :1225-1229 */
+                if (hook) hook(ctx, OR_APR_EV_MU, iter, n, inner, U, lambda, Ph, kv);
             }
+            /* KruskalModelNormalizeMode, :1243 */
+#if OR_MUTANT == 3
+            for (uint64_t r = 0; r < R; ++r) { /* 2-norms into lambda: the wrong normalisation */
+                double c = 0.0;
+                for (uint64_t i = 0; i < dim; ++i) c += B[i * R + r] * B[i * R + r];
+                c = sqrt(c);
+                for (uint64_t i = 0; i < dim; ++i) B[i * R + r] /= c;
+                lambda[r] *= c;
+            }
+#else
             or_normalize_mode_l1(dim, R, B, lambda);
+#endif
+            if (hook) hook(ctx, OR_APR_EV_NORM, iter, n, -1, U, lambda, Ph, kkt_mode[n]);
         }
+        /* :1248-1252 and the outer stop :1261 (kkt[-1] at iter 0 is read as "no stop") */
         kkt[iter] = kkt_mode[0];
         for (int n = 1; n < N; ++n) kkt[iter] = fmax(kkt[iter], kkt_mode[n]);
+        if (hook) hook(ctx, OR_APR_EV_OUTER, iter, -1, -1, U, lambda, NULL, kkt[iter]);
+#if OR_MUTANT == 4
+        if (iter > 0 && fabs(kkt[iter] - kkt[iter - 1]) > eps * eps && !dobench) converged = 1;
+#else
         if (iter > 0 && fabs(kkt[iter] - kkt[iter - 1]) <= eps * eps && !dobench) converged = 1;
+#endif
         if (converged) break; /* as the reference: the converged iteration is not counted */
     }
     int total = 0;
@@ -1045,4 +1089,11 @@ int or_cp_apr(or_alto *at, uint64_t R, int max_iters, int max_inner, double eps,
     for (int n = 0; n < N; ++n) { free(grams[n]); free(Phi[n]); }
     free(tm); free(kkt); free(n_inner);
     return iter;
+}
+
+int or_cp_apr(or_alto *at, uint64_t R, int max_iters, int max_inner, double eps, double eps_div_zero,
+              double kappa_tol, double kappa, int dobench, double **U, double *lambda, or_apr_result *res)
+{
+    return or_cp_apr_hooked(at, R, max_iters, max_inner, eps, eps_div_zero, kappa_tol, kappa, dobench, U, lambda,
+                            res, NULL, NULL);
 }

@@ -105,6 +105,18 @@ int or_cp_apr(or_alto *at, uint64_t rank, int max_iters, int max_inner, double e
               double eps_div_zero, double kappa_tol, double kappa, int dobench,
               double **U, double *lambda, or_apr_result *res);
 
+/* Observer of the CP-APR driver's steps (tests only).  U / lambda are the
+ * live model (U[mode] is B); phi is Phi of `mode` (at PRE_FIX / POST_FIX:
+ * the Phi the kappa fix reads, from the previous outer iteration); kkt is
+ * the mode's KKT value (PHI, MU, NORM) or the outer maximum (OUTER). */
+enum { OR_APR_EV_START = 0, OR_APR_EV_PRE_FIX = 1, OR_APR_EV_POST_FIX = 2, OR_APR_EV_PHI = 3,
+       OR_APR_EV_MU = 4, OR_APR_EV_NORM = 5, OR_APR_EV_OUTER = 6 };
+typedef void (*or_apr_hook)(void *ctx, int event, int iter, int mode, int inner, double **U,
+                            const double *lambda, const double *phi, double kkt);
+int or_cp_apr_hooked(or_alto *at, uint64_t rank, int max_iters, int max_inner, double epsilon,
+                     double eps_div_zero, double kappa_tol, double kappa, int dobench,
+                     double **U, double *lambda, or_apr_result *res, or_apr_hook hook, void *ctx);
+
 /* norm of the tensor values, sum of squares */
 double or_normsq(const or_alto *at);
 
