@@ -54,6 +54,38 @@ class AprResult(C.Structure):
                 ("t_apr", C.c_double), ("t_phi", C.c_double), ("t_post", C.c_double)]
 
 
+def _one_hip_runtime():
+    """Make this library and PyTorch share one HIP runtime, whichever of the
+    two is imported first.
+
+    PyTorch's ROCm wheel ships its own libamdhip64.so (soname
+    libamdhip64.so.7), which its libraries request as "libamdhip64.so";
+    libalto_hip.so requests "libamdhip64.so.7".  If this library were loaded
+    first, the loader would map /opt/rocm's runtime for it and later torch's
+    copy for torch (the names differ), and two HIP runtimes in one process
+    do not share the GPU.  Loading torch's copy first, by path and with
+    global symbols, makes both requests resolve to it: ours by soname, torch's
+    by file identity.  Without PyTorch installed, /opt/rocm's runtime is
+    used."""
+    import importlib.util
+
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+    torch = sys.modules.get("torch")
+    if torch is not None:  # already imported: let it initialise the runtime first
+        try:
+            torch.cuda.is_available()
+        except Exception:
+            pass
+
+
 def lib():
     """Load libalto_hip.so; raises if it was not built (no fallback)."""
     global _lib
@@ -61,15 +93,7 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise AltoError(f"{LIB_PATH} is missing: build it with `python -m alto_amd.build`")
-    # PyTorch's ROCm wheels carry a HIP runtime of their own, which finds no
-    # GPU when the other runtime in the process initialised first: when torch
-    # is loaded, let it start before this library makes its first HIP call
-    torch = sys.modules.get("torch")
-    if torch is not None:
-        try:
-            torch.cuda.is_available()
-        except Exception:
-            pass
+    _one_hip_runtime()
     L = C.CDLL(LIB_PATH)
     vp, u64, i32, f64 = C.c_void_p, C.c_uint64, C.c_int, C.c_double
     sig = {

@@ -5,6 +5,7 @@
 #include <hip/hip_bf16.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -141,6 +142,16 @@ struct alto_tensor {
         int mode, rank, precision, variant, seg;
     };
     std::vector<Tuned> tuned;
+
+    // Streams that asynchronous calls enqueued work on for this tensor, each
+    // with an event recorded behind that work: the tensor's buffers go back
+    // to the shared device cache (destroy, re-ordered streams, grown copies)
+    // only after every such event has completed.
+    std::mutex use_mu;
+    std::vector<std::pair<hipStream_t, hipEvent_t>> uses;
+    int note_use(hipStream_t s);  // event behind the work enqueued so far on s
+    int wait_uses();              // host wait for every recorded event
+    ~alto_tensor();
 };
 
 namespace alto {

@@ -24,44 +24,55 @@
 
 namespace alto {
 
+// Per-phase event timing of eager iterations.  The events are created once
+// and reused by every later iteration (a mark is one hipEventRecord); they
+// are destroyed with the timer, whose owner has drained the stream first.
 struct PhaseTimer {
     bool on = false;
     hipStream_t s = nullptr;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> phase;
+    std::vector<hipEvent_t> ev;  // created on demand, kept
+    std::vector<int> phase;      // phase of mark k (-1: none)
+    size_t used = 0;             // marks since the last collect
     int init(bool enable, hipStream_t st)
     {
         on = enable;
         s = st;
+        used = 0;
         return ALTO_OK;
     }
     int mark(int ph)
     {
         if (!on) return ALTO_OK;
-        hipEvent_t e;
-        HIP_TRY(hipEventCreate(&e));
-        HIP_TRY(hipEventRecord(e, s));
-        ev.push_back(e);
-        phase.push_back(ph);
+        if (used == ev.size()) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreate(&e));
+            ev.push_back(e);
+            phase.push_back(-1);
+        }
+        HIP_TRY(hipEventRecord(ev[used], s));
+        phase[used] = ph;
+        ++used;
         return ALTO_OK;
     }
-    // accumulate the time between consecutive marks into the phase of the later mark
+    // accumulate the time between consecutive marks into the phase of the
+    // later mark (the stream has been synchronized by the caller)
     int collect(double* acc)
     {
         if (!on) return ALTO_OK;
-        for (size_t k = 1; k < ev.size(); ++k) {
+        for (size_t k = 1; k < used; ++k) {
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, ev[k - 1], ev[k]));
             if (phase[k] >= 0) acc[phase[k]] += ms * 1e-3;
         }
-        for (auto e : ev) (void)hipEventDestroy(e);
-        ev.clear();
-        phase.clear();
+        used = 0;
         return ALTO_OK;
     }
     ~PhaseTimer()
     {
-        for (auto e : ev) (void)hipEventDestroy(e);
+        for (auto e : ev) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
     }
 };
 
@@ -244,7 +255,23 @@ struct CpdSessionBase {
     hipStream_t s = nullptr;
     double acc[PH_N] = {0};
     double total = 0;
-    virtual ~CpdSessionBase() {}
+    PhaseTimer pt;  // events reused by every eager iteration
+    // event behind the last work this session enqueued (on any stream):
+    // destruction waits for it before the buffers go back to the cache
+    hipEvent_t last = nullptr;
+    int fence(hipStream_t st)
+    {
+        if (!last) HIP_TRY(hipEventCreateWithFlags(&last, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(last, st));
+        return ALTO_OK;
+    }
+    virtual ~CpdSessionBase()
+    {
+        if (last) {
+            (void)hipEventSynchronize(last);
+            (void)hipEventDestroy(last);
+        }
+    }
     virtual int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) = 0;
     virtual int get(double* const* Uh, double* lambda_h) = 0;
     // pieces of one eager iteration, for iterating several sessions in
@@ -270,6 +297,7 @@ struct CpdSession : CpdSessionBase {
     hipStream_t cs = nullptr;
     ~CpdSession() override
     {
+        if (last) (void)hipEventSynchronize(last);  // before the buffers (members) are released
         for (auto& e : exec)
             if (e) (void)hipGraphExecDestroy(e);
         if (fit_host) (void)hipHostFree(fit_host);
@@ -327,6 +355,7 @@ struct CpdSession : CpdSessionBase {
         ALTO_TRY(fill_value<MT>(c.M.template as<MT>(), t->dims[0] * R, (MT)0, s));  // the first MTTKRP accumulates
         ALTO_TRY(normsq_device(t->vals.as<double>(), t->nnz, c.nxpart.template as<double>(), c.nx.template as<double>(), s));
         HIP_TRY(hipMemcpyAsync(&normX, c.nx.p, 8, hipMemcpyDeviceToHost, s));
+        ALTO_TRY(fence(s));
         HIP_TRY(hipStreamSynchronize(s));
         return ALTO_OK;
     }
@@ -338,20 +367,20 @@ struct CpdSession : CpdSessionBase {
     }
     int mode_rest(int j, hipStream_t st) override
     {
-        PhaseTimer pt;
-        ALTO_TRY(pt.init(false, st));
-        return cpd_mode_rest(t, c, j, it, pt, st);
+        PhaseTimer off;
+        ALTO_TRY(off.init(false, st));
+        ALTO_TRY(cpd_mode_rest(t, c, j, it, off, st));
+        return fence(st);
     }
     int fit_launch(hipStream_t st, double* fit_dst) override
     {
         ALTO_TRY(cpd_fit_step(t, c, normX, st));
         HIP_TRY(hipMemcpyAsync(fit_dst, c.fitbuf.p, sizeof(double), hipMemcpyDeviceToHost, st));
-        return ALTO_OK;
+        return fence(st);
     }
     int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) override
     {
         if (graphs_enabled() && !verbose) return iterate_graph(max_iters, eps, fit_trace, ran);
-        PhaseTimer pt;
         ALTO_TRY(pt.init(timed || verbose, s));
         int done = 0;
         const double t0 = now_s();
@@ -364,6 +393,7 @@ struct CpdSession : CpdSessionBase {
             ALTO_TRY(pt.mark(PH_FIT));
             double fb[3];
             HIP_TRY(hipMemcpyAsync(fb, c.fitbuf.p, 24, hipMemcpyDeviceToHost, s));
+            ALTO_TRY(fence(s));
             HIP_TRY(hipStreamSynchronize(s));
             ALTO_TRY(pt.collect(acc));
             fit = fb[0];
@@ -387,6 +417,7 @@ struct CpdSession : CpdSessionBase {
             const int kind = it == 0 ? 0 : 1;
             if (!exec[kind]) ALTO_TRY(capture(kind));
             HIP_TRY(hipGraphLaunch(exec[kind], s));
+            ALTO_TRY(fence(s));
             HIP_TRY(hipStreamSynchronize(s));
             fit = fit_host[0];
             if (fit_trace) fit_trace[k] = fit;
@@ -622,26 +653,27 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
     // the sessions' solve chains are independent: they run on NAUX private
     // streams, forked from `s` after each batched MTTKRP and joined back
     // before the next one (the next MTTKRP accumulates into the M that each
-    // chain's last launch clears)
+    // chain's last launch clears).  The streams and events are created once
+    // per thread and device and kept until the process ends.
     constexpr int NAUX = 4;
     struct Aux {
+        int device = -1;
         hipStream_t st[NAUX] = {};
         hipEvent_t fork = nullptr, join[NAUX] = {};
-        ~Aux()
-        {
-            for (auto x : st)
-                if (x) (void)hipStreamDestroy(x);
-            for (auto e : join)
-                if (e) (void)hipEventDestroy(e);
-            if (fork) (void)hipEventDestroy(fork);
-        }
-    } aux;
+    };
+    static thread_local Aux aux_cache[8];
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    Aux& aux = aux_cache[dev & 7];
     const int naux = n > 1 ? NAUX : 0;
-    for (int k = 0; k < naux; ++k) {
-        HIP_TRY(hipStreamCreateWithFlags(&aux.st[k], hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&aux.join[k], hipEventDisableTiming));
+    if (naux && aux.device != dev) {
+        for (int k = 0; k < NAUX; ++k) {
+            HIP_TRY(hipStreamCreateWithFlags(&aux.st[k], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&aux.join[k], hipEventDisableTiming));
+        }
+        HIP_TRY(hipEventCreateWithFlags(&aux.fork, hipEventDisableTiming));
+        aux.device = dev;
     }
-    if (naux) HIP_TRY(hipEventCreateWithFlags(&aux.fork, hipEventDisableTiming));
     auto fork = [&]() -> int {
         if (!naux) return ALTO_OK;
         HIP_TRY(hipEventRecord(aux.fork, s));
@@ -687,8 +719,8 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
                     outs.push_back(M);
                 }
             // M is zero here (cleared at set-up and by each mode's last launch)
-            ALTO_TRY(alto_mttkrp_batched((int)ts.size(), ts.data(), modes.data(), (uint64_t)b[0]->R, fpp.data(),
-                                         outs.data(), b[0]->prec, 1, stream));
+            ALTO_TRY(mttkrp_batched_impl((int)ts.size(), ts.data(), modes.data(), (uint64_t)b[0]->R, fpp.data(),
+                                         outs.data(), b[0]->prec, 1, s));
             ALTO_TRY(fork());
             int q = 0;
             for (int i : act)

@@ -131,6 +131,39 @@ size_t pool_in_use(int dev)
     return P.in_use[dev & 63];
 }
 
+}  // namespace alto
+
+int alto_tensor::note_use(hipStream_t s)
+{
+    std::lock_guard<std::mutex> g(use_mu);
+    hipEvent_t e = nullptr;
+    for (auto& u : uses)
+        if (u.first == s) e = u.second;
+    if (!e) {
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        uses.emplace_back(s, e);
+    }
+    HIP_TRY(hipEventRecord(e, s));
+    return ALTO_OK;
+}
+
+int alto_tensor::wait_uses()
+{
+    std::lock_guard<std::mutex> g(use_mu);
+    for (auto& u : uses) HIP_TRY(hipEventSynchronize(u.second));
+    return ALTO_OK;
+}
+
+alto_tensor::~alto_tensor()
+{
+    for (auto& u : uses) {
+        (void)hipEventSynchronize(u.second);
+        (void)hipEventDestroy(u.second);
+    }
+}
+
+namespace alto {
+
 // ------------------------------------------------------------------ masks
 // setup_packed_alto(at, LSB_FIRST, SHORT_FIRST): modes ordered by bit count
 // (stable), one bit per still-active mode per level, LSB first.
@@ -865,6 +898,7 @@ int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStr
         if (t->pcopy[m].bytes >= need) continue;
         if (!drained) {  // the old copy goes back to the shared cache
             HIP_TRY(hipStreamSynchronize(s));
+            ALTO_TRY(t->wait_uses());
             drained = true;
         }
         t->pcopy[m].release();
@@ -1006,6 +1040,7 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     if (!any && !reorder) return ALTO_OK;
     // buffers go back to the shared cache: drain work that may still read them
     HIP_TRY(hipStreamSynchronize(s));
+    ALTO_TRY(t->wait_uses());
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
         if (M.chunk_log2 != t->chunk_log2 || M.ranked != ranked) {
@@ -1227,8 +1262,10 @@ int alto_tensor_create(int nmodes, const uint64_t* dims, uint64_t nnz, const uin
 
 int alto_tensor_destroy(alto_tensor_t* t)
 {
+    if (!t) return ALTO_OK;
+    const int st = t->wait_uses();  // kernels of asynchronous calls may still read the buffers
     delete t;
-    return ALTO_OK;
+    return st;
 }
 
 int alto_tensor_info(const alto_tensor_t* t, alto_tensor_info_t* info)
@@ -1317,7 +1354,7 @@ int alto_pdep_keys(const alto_tensor_t* t, const uint64_t* coords, uint64_t n, u
                            lutref(t->lin), n, keys);
     });
     HIP_TRY(hipGetLastError());
-    return ALTO_OK;
+    return const_cast<alto_tensor*>(t)->note_use(s);
 }
 
 int alto_pext_keys(const alto_tensor_t* t, const uint64_t* keys, uint64_t n, uint64_t* coords, alto_stream_t stream)
@@ -1332,7 +1369,7 @@ int alto_pext_keys(const alto_tensor_t* t, const uint64_t* keys, uint64_t n, uin
                            lutref(t->delin), t->pack, n, coords);
     });
     HIP_TRY(hipGetLastError());
-    return ALTO_OK;
+    return const_cast<alto_tensor*>(t)->note_use(s);
 }
 
 int alto_device_malloc(void** p, uint64_t bytes)

@@ -340,6 +340,7 @@ template int gram_device<double>(const double*, uint64_t, int, double*, double*,
 template int gram_device<float>(const float*, uint64_t, int, double*, double*, hipStream_t, bool);
 template int gram_device<bf16_t>(const bf16_t*, uint64_t, int, double*, double*, hipStream_t, bool);
 
+
 // ------------------------------------------------------------------ V and its Cholesky factor
 // V^-1 for symmetric positive definite V = prod_{m != mode} G_m by in-place
 // Gauss-Jordan elimination without pivoting (one workgroup, R <= 128): the

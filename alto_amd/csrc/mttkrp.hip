@@ -537,12 +537,33 @@ int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* fa
         set_error("alto_mttkrp: null argument");
         return ALTO_ERR_ARG;
     }
-    return mttkrp_device(t, mode, (int)rank, factors, out, precision, accumulate, (hipStream_t)stream, 0);
+    ALTO_TRY(mttkrp_device(t, mode, (int)rank, factors, out, precision, accumulate, (hipStream_t)stream, 0));
+    return t->note_use((hipStream_t)stream);
 }
 
 int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* modes, uint64_t rank,
                         const void* const* const* factors, void* const* outs, int precision, int accumulate,
                         alto_stream_t stream)
+{
+    ALTO_TRY(alto::mttkrp_batched_impl(njobs, tensors, modes, rank, factors, outs, precision, accumulate,
+                                       (hipStream_t)stream));
+    for (int i = 0; i < njobs; ++i) {
+        bool seen = false;  // one event per tensor listed several times
+        for (int k = 0; k < i && !seen; ++k) seen = tensors[k] == tensors[i];
+        if (!seen) ALTO_TRY(tensors[i]->note_use((hipStream_t)stream));
+    }
+    return ALTO_OK;
+}
+
+}  // extern "C"
+
+namespace alto {
+
+// alto_mttkrp_batched without the tensors' use events (for callers that
+// synchronize the stream themselves, such as lockstep CP-ALS)
+int mttkrp_batched_impl(int njobs, alto_tensor_t* const* tensors, const int* modes, uint64_t rank,
+                        const void* const* const* factors, void* const* outs, int precision, int accumulate,
+                        hipStream_t s)
 {
     if (njobs < 0 || (njobs > 0 && (!tensors || !modes || !factors || !outs))) {
         set_error("alto_mttkrp_batched: null argument");
@@ -553,7 +574,6 @@ int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* mod
         return ALTO_ERR_ARG;
     }
     const int R = (int)rank;
-    hipStream_t s = (hipStream_t)stream;
     for (int i = 0; i < njobs; ++i) {
         alto_tensor* t = tensors[i];
         if (!t || !factors[i] || !outs[i]) {
@@ -602,6 +622,10 @@ int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* mod
                                     (const float* const*)vals.data(), (float* const*)outs, g_seg, k, s);
 }
 
+}  // namespace alto
+
+extern "C" {
+
 int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* factors, double* phi, double eps_div_zero,
              alto_stream_t stream)
 {
@@ -609,7 +633,8 @@ int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* fac
         set_error("alto_phi: null argument");
         return ALTO_ERR_ARG;
     }
-    return phi_device(t, mode, (int)rank, factors, phi, eps_div_zero, (hipStream_t)stream, 0);
+    ALTO_TRY(phi_device(t, mode, (int)rank, factors, phi, eps_div_zero, (hipStream_t)stream, 0));
+    return t->note_use((hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -107,7 +107,9 @@ struct alto_server {
         return nullptr;
     }
 
-    void run_job(Job* j, hipStream_t s)
+    // runs the job; returns its final status, which the caller publishes
+    // under the server's lock after every result field is written
+    int run_job(Job* j, hipStream_t s)
     {
         j->t_start = now_s();
         const bool dev = j->dev_vals != nullptr;
@@ -141,7 +143,7 @@ struct alto_server {
         std::vector<std::vector<uint64_t>>().swap(j->cidx);
         std::vector<double>().swap(j->vals);
         j->t_end = now_s();
-        j->status = st == ALTO_OK ? 2 : st;
+        return st == ALTO_OK ? 2 : st;
     }
 
     void worker_main()
@@ -156,9 +158,10 @@ struct alto_server {
                 cv_work.wait(lk, [&] { return stop || (j = admit_locked()) != nullptr; });
                 if (!j) break;
             }
-            run_job(j, s);
+            const int status = run_job(j, s);
             {
                 std::lock_guard<std::mutex> g(mu);
+                j->status = status;
                 in_use -= j->footprint;
                 --running;
             }
@@ -249,7 +252,8 @@ int alto_server_release(alto_server_t* S, uint64_t job_id)
     if (!S) return ALTO_ERR_ARG;
     std::lock_guard<std::mutex> g(S->mu);
     auto it = S->jobs.find(job_id);
-    if (it == S->jobs.end() || it->second->status < 2) return ALTO_ERR_BUSY;
+    if (it == S->jobs.end()) return ALTO_ERR_ARG;
+    if (it->second->status == 0 || it->second->status == 1) return ALTO_ERR_BUSY;  // queued or running
     S->jobs.erase(it);
     return ALTO_OK;
 }
