@@ -114,6 +114,7 @@ def lib():
         "alto_cpd_fit": (i32, [vp, u64, vp, vp, C.POINTER(f64)]),
         "alto_cp_apr": (i32, [vp, u64, i32, i32, f64, f64, f64, f64, i32, vp, vp, C.POINTER(AprResult), i32, vp]),
         "alto_update_gram": (i32, [vp, u64, u64, i32, vp, vp]),
+        "alto_update_gram_noalloc": (i32, [vp, u64, u64, i32, vp, vp]),
         "alto_cpd_session_create": (i32, [vp, u64, i32, vp, vp, C.POINTER(vp)]),
         "alto_cpd_session_create_seeded": (i32, [vp, u64, i32, u64, vp, C.POINTER(vp)]),
         "alto_cpd_session_iterate": (i32, [vp, i32, f64, vp, C.POINTER(i32), C.POINTER(i32), i32]),

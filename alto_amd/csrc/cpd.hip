@@ -825,4 +825,14 @@ int alto_update_gram(const void* U, uint64_t dim, uint64_t rank, int precision, 
     return ALTO_OK;
 }
 
+int alto_update_gram_noalloc(const void* U, uint64_t dim, uint64_t rank, int precision, double* gram,
+                             alto_stream_t stream)
+{
+    if (rank > 128) {
+        set_error("alto_update_gram_noalloc: rank %llu above 128", (unsigned long long)rank);
+        return ALTO_ERR_ARG;
+    }
+    return gram_noalloc(U, dim, (int)rank, precision, gram, (hipStream_t)stream);
+}
+
 }  // extern "C"

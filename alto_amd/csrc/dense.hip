@@ -17,6 +17,7 @@
 //    cpd_fit_alto (include/cpd.hpp:189-271).
 #include <math.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "alto_internal.hpp"
@@ -340,6 +341,56 @@ template int gram_device<double>(const double*, uint64_t, int, double*, double*,
 template int gram_device<float>(const float*, uint64_t, int, double*, double*, hipStream_t, bool);
 template int gram_device<bf16_t>(const bf16_t*, uint64_t, int, double*, double*, hipStream_t, bool);
 
+// ------------------------------------------------------------------ Gram without allocation
+// Per-block partial Grams of alto_update_gram_noalloc live in this array:
+// one copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: the device address comes from
+// hipGetSymbolAddress, resolved once per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 16u << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static int scratch_address(double** out)
+{
+    static std::mutex mu;
+    static double* addr[64] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> g(mu);
+    if (!addr[dev & 63]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        addr[dev & 63] = static_cast<double*>(p);
+    }
+    *out = addr[dev & 63];
+    return ALTO_OK;
+}
+
+// update_gram into a device R x R FP64 matrix with the block partials in the
+// per-device scratch array; asynchronous on `s`.  The blocks are the ones
+// alto_update_gram uses, so the two agree bit for bit.
+int gram_noalloc(const void* U, uint64_t dim, int R, int precision, double* gram, hipStream_t s)
+{
+    if (!U || !gram || R < 1 || R > 128 || dim < 1) {
+        set_error("alto_update_gram_noalloc: invalid argument (rank 1..128, dim >= 1, non-null pointers)");
+        return ALTO_ERR_ARG;
+    }
+    if (precision != ALTO_PREC_F64 && precision != ALTO_PREC_F32 && precision != ALTO_PREC_BF16) {
+        set_error("alto_update_gram_noalloc: unknown precision %d", precision);
+        return ALTO_ERR_ARG;
+    }
+    const size_t need = (size_t)gram_blocks(dim) * (size_t)R * (size_t)R * sizeof(double);
+    if (need > NOALLOC_SCRATCH_BYTES) {
+        set_error("alto_update_gram_noalloc: %zu bytes of partial Grams (dim %llu, rank %d) exceed the %zu-byte "
+                  "scratch; use alto_update_gram",
+                  need, (unsigned long long)dim, R, NOALLOC_SCRATCH_BYTES);
+        return ALTO_ERR_UNSUPPORTED;
+    }
+    double* part = nullptr;
+    ALTO_TRY(scratch_address(&part));
+    if (precision == ALTO_PREC_F64) return gram_device<double>((const double*)U, dim, R, gram, part, s, true);
+    if (precision == ALTO_PREC_F32) return gram_device<float>((const float*)U, dim, R, gram, part, s, false);
+    return gram_device<bf16_t>((const bf16_t*)U, dim, R, gram, part, s, false);
+}
 
 // ------------------------------------------------------------------ V and its Cholesky factor
 // V^-1 for symmetric positive definite V = prod_{m != mode} G_m by in-place

@@ -35,5 +35,7 @@ int fit_scalar_device(const double* grams, int nmode, int R, const double* lambd
                       double normX, double* out, hipStream_t s);
 int random_factor_device(double* U, uint64_t n, uint64_t seed, int mode, hipStream_t s);
 int normsq_device(const double* v, uint64_t n, double* part, double* out, hipStream_t s);
+// alto_update_gram_noalloc (partials in a per-device scratch array)
+int gram_noalloc(const void* U, uint64_t dim, int R, int precision, double* gram, hipStream_t s);
 
 }  // namespace alto

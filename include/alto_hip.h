@@ -245,6 +245,20 @@ int alto_pseudo_inverse(const double* grams, int nmodes, int mode, uint64_t rank
 int alto_update_gram(const void* U, uint64_t dim, uint64_t rank, int precision, double* gram,
                      alto_stream_t stream);
 
+/* alto_update_gram without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-block partial Grams live in a static 16 MiB device
+ * array that each GPU holds until the process ends; sizes whose partials
+ * (one rank x rank FP64 matrix per 256 rows, at most 512 of them) exceed it
+ * are rejected with ALTO_ERR_UNSUPPORTED (ranks up to 64 always fit; rank
+ * 128 up to 32768 rows).  Asynchronous on `stream`; the result equals
+ * alto_update_gram's bit for bit.  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (different streams or threads)
+ * race: order them on one stream or synchronize between them. */
+int alto_update_gram_noalloc(const void* U, uint64_t dim, uint64_t rank, int precision, double* gram,
+                             alto_stream_t stream);
+
 /* ---- CP-APR --------------------------------------------------------------- */
 
 typedef struct {
