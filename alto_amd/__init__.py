@@ -297,7 +297,7 @@ class AltoTensor:
                 keep.append(a)
             factor_ptrs = [a.ptr for a in keep]
         if out_ptr is None:
-            o = DeviceArray((int(max(self.dims)), rank), np.float64 if precision == PREC_F64 else np.float32)
+            o = DeviceArray((int(max(self.dims)), rank), np.float64)
             keep.append(o)
             out_ptr = o.ptr
         res = (TuneEntry * self.nmodes)()
@@ -313,10 +313,10 @@ class AltoTensor:
             out = DeviceArray((int(self.dims[mode]), R), np.float64)
         elif precision == PREC_F32:
             dev = [DeviceArray.from_numpy(np.asarray(f, np.float32)) for f in factors]
-            out = DeviceArray((int(self.dims[mode]), R), np.float32)
+            out = DeviceArray((int(self.dims[mode]), R), np.float64)
         else:
             dev = [DeviceArray.from_numpy(to_bf16_bits(f)) for f in factors]
-            out = DeviceArray((int(self.dims[mode]), R), np.float32)
+            out = DeviceArray((int(self.dims[mode]), R), np.float64)
         self.mttkrp_device(mode, dev, out, precision)
         synchronize()
         return out.numpy()
@@ -428,7 +428,7 @@ def mttkrp_batched(tensors, modes, factors, precision=PREC_F64):
         keep.append((dev, arr))
         fptrs.append(C.cast(arr, C.c_void_p).value)
         rows = int(t.dims[m]) if 0 <= m < len(t.dims) else 1  # a bad mode is the library's to reject
-        outs.append(DeviceArray((rows, R), np.float64 if precision == PREC_F64 else np.float32))
+        outs.append(DeviceArray((rows, R), np.float64))
     th = _ptr_array([t.h for t in tensors])
     md = (C.c_int * n)(*modes)
     check(lib().alto_mttkrp_batched(n, th, md, R, _ptr_array(fptrs), _ptr_array([o.ptr for o in outs]), precision,

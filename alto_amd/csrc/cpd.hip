@@ -457,13 +457,13 @@ static int session_new(alto_tensor* t, int R, int prec, double* const* Uh, hipSt
         break;
     }
     case ALTO_PREC_F32: {
-        auto* x = new CpdSession<float, float, float>();
+        auto* x = new CpdSession<float, float, double>();
         st = x->init(t, R, prec, Uh, s, dev_seed);
         b = x;
         break;
     }
     case ALTO_PREC_BF16: {
-        auto* x = new CpdSession<float, bf16_t, float>();
+        auto* x = new CpdSession<float, bf16_t, double>();
         st = x->init(t, R, prec, Uh, s, dev_seed);
         b = x;
         break;

@@ -633,9 +633,10 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
                                                UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
                                                double* __restrict__ pin)
 {
-    // FP32 / BF16 models (M and U in FP32): the products on FP32 MFMA with
-    // the FP64 inverse rounded to FP32; FP64 models: FP64 MFMA
-    constexpr bool F32 = std::is_same<MT, float>::value && std::is_same<UT, float>::value;
+    // FP32 / BF16 models (U in FP32; M in FP64 or FP32): the products on
+    // FP32 MFMA with M and the FP64 inverse rounded to FP32; FP64 models:
+    // FP64 MFMA
+    constexpr bool F32 = std::is_same<UT, float>::value;
     using mma_t = typename std::conditional<F32, float, double>::type;
     using acc_t = typename std::conditional<F32, f4_t, d4_t>::type;
     constexpr int KS = 4 * T16;  // k-steps of 4 covering 16 T16 >= R columns
@@ -928,8 +929,8 @@ int solve_device(const MT* M, uint64_t I, int R, const double* L, const double* 
 }
 template int solve_device<double, double>(const double*, uint64_t, int, const double*, const double*, const int*,
                                           double*, double*, double*, int, hipStream_t, double*);
-template int solve_device<float, float>(const float*, uint64_t, int, const double*, const double*, const int*, float*,
-                                        double*, double*, int, hipStream_t, double*);
+template int solve_device<double, float>(const double*, uint64_t, int, const double*, const double*, const int*, float*,
+                                         double*, double*, int, hipStream_t, double*);
 
 int lambda_device(const double* psum, const double* pmax, int nb, int R, int which, double* lambda, hipStream_t s)
 {

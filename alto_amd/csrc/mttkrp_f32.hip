@@ -3,16 +3,16 @@
 
 namespace alto {
 
-int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const* U, const float* val, float* out,
+int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const* U, const float* val, double* out,
                          int seg, const MttkrpTune& k, hipStream_t s)
 {
-    return launch_mttkrp<float, float, float, float>(t, mode, R, U, val, out, seg, k, s);
+    return launch_mttkrp<float, float, float, double>(t, mode, R, U, val, out, seg, k, s);
 }
 
 int launch_mttkrp_batch_f32(int n, alto_tensor* const* ts, const int* modes, int R, const float* const* const* U,
-                            const float* const* vals, float* const* outs, int seg, const MttkrpTune& k, hipStream_t s)
+                            const float* const* vals, double* const* outs, int seg, const MttkrpTune& k, hipStream_t s)
 {
-    return launch_mttkrp_batch<float, float, float, float>(n, ts, modes, R, U, vals, outs, seg, k, s);
+    return launch_mttkrp_batch<float, float, float, double>(n, ts, modes, R, U, vals, outs, seg, k, s);
 }
 
 }  // namespace alto

@@ -782,15 +782,15 @@ int launch_mttkrp_batch(int n, alto_tensor* const* ts, const int* modes, int R, 
 
 int launch_mttkrp_f64(const alto_tensor* t, int mode, int R, const double* const* U, const double* val, double* out,
                       int seg, const MttkrpTune& k, hipStream_t s);
-int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const* U, const float* val, float* out,
+int launch_mttkrp_f32(const alto_tensor* t, int mode, int R, const float* const* U, const float* val, double* out,
                       int seg, const MttkrpTune& k, hipStream_t s);
-int launch_mttkrp_bf16(const alto_tensor* t, int mode, int R, const bf16_t* const* U, const float* val, float* out,
+int launch_mttkrp_bf16(const alto_tensor* t, int mode, int R, const bf16_t* const* U, const float* val, double* out,
                        int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_batch_f64(int n, alto_tensor* const* ts, const int* modes, int R, const double* const* const* U,
                             const double* const* vals, double* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_batch_f32(int n, alto_tensor* const* ts, const int* modes, int R, const float* const* const* U,
-                            const float* const* vals, float* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
+                            const float* const* vals, double* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
 int launch_mttkrp_batch_bf16(int n, alto_tensor* const* ts, const int* modes, int R, const bf16_t* const* const* U,
-                            const float* const* vals, float* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
+                            const float* const* vals, double* const* outs, int seg, const MttkrpTune& k, hipStream_t s);
 
 }  // namespace alto

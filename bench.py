@@ -302,7 +302,7 @@ def precision_leg():
         ptr = A._ptr_array([f.data_ptr() for f in F])
         per = []
         for m in range(len(dims)):
-            out = torch.zeros(dims[m], R, dtype=torch.float64 if prec == A.PREC_F64 else torch.float32, device="cuda")
+            out = torch.zeros(dims[m], R, dtype=torch.float64, device="cuda")
             A.check(A.lib().alto_mttkrp(T.h, m, R, ptr, out.data_ptr(), prec, 0, stream), "mttkrp")
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
@@ -432,7 +432,7 @@ def batched_leg(n_jobs=16, reps=50):
         for t, dims, _ in tensors:
             fs = [torch.rand((d, R), dtype=dt, device="cuda") for d in dims]
             arr = A._ptr_array([f.data_ptr() for f in fs])
-            o = torch.zeros((dims[0], R), dtype=dt, device="cuda")
+            o = torch.zeros((dims[0], R), dtype=torch.float64, device="cuda")
             keep.append((fs, arr, o))
             fptrs.append(arr)
             outs.append(o)

@@ -100,9 +100,10 @@ int alto_pext_keys(const alto_tensor_t* t, const uint64_t* keys, uint64_t n, uin
 /* mttkrp_alto_par(target_mode, factors, rank, at, wlocks, ofibs)
  * (include/alto.hpp:617-623).  factors: nmodes device pointers to row-major
  * I_n x rank matrices of the precision's storage type (double / float /
- * bf16).  out: device I_mode x rank, double for ALTO_PREC_F64, float
- * otherwise.  accumulate != 0 adds into out as the reference does; 0
- * overwrites it.  Asynchronous on `stream`. */
+ * bf16).  out: device I_mode x rank doubles in every precision (FP32 / BF16:
+ * products and runs of one output row summed in FP32 registers, the row
+ * partials added into the FP64 output).  accumulate != 0 adds into out as
+ * the reference does; 0 overwrites it.  Asynchronous on `stream`. */
 int alto_mttkrp(alto_tensor_t* t, int mode, uint64_t rank, const void* const* factors, void* out,
                 int precision, int accumulate, alto_stream_t stream);
 
@@ -164,8 +165,7 @@ int alto_tune_set(int param, int value);
  * -- CP-ALS sessions included -- use it while ALTO_TUNE_SEG and
  * ALTO_TUNE_MTTKRP_VARIANT are at their defaults.  `factors` are device
  * factor matrices of that precision; `out` is device scratch of
- * max_n dims[n] x rank elements of the output type (FP64, or FP32 for FP32
- * and BF16), overwritten.  `result` (may be NULL) receives nmode entries
+ * max_n dims[n] x rank doubles, overwritten.  `result` (may be NULL) receives nmode entries
  * (variant -1 / seg 0: the defaults were fastest).  Synchronous. */
 typedef struct {
     int mode;
