@@ -96,6 +96,15 @@ struct ModeStream {
     int split = 0;
     int hot_k[ALTO_MAXN - 1] = {0};
     std::vector<uint64_t> pat_start;
+    // Fiber layout (fib_mode >= 0): the stream is ordered by (chunk, row of
+    // this mode, row of mode fib_mode) and split in two parts that each keep
+    // that order: first the nonzeros of fibers -- runs of equal (chunk, row,
+    // fiber-mode row) -- of at least fib_min nonzeros (nlong of them), then
+    // the others.  MTTKRP reads the fiber-mode row once per fiber in the
+    // first part.
+    int fib_mode = -1;
+    int fib_min = 0;
+    uint64_t nlong = 0;
 };
 
 }  // namespace alto
@@ -168,6 +177,11 @@ int hot_rows_enabled();
 int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStream_t s);
 int split_stream(alto_tensor* t, int mode, const int* hot_k, hipStream_t s);
 int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s);
+// fiber layout settings (mttkrp.hip): 0 when off; else the minimum fiber
+// length of the first part, the chunk size and the smallest tensor that
+// gets the layout
+int fiber_settings(int* min_len, int* chunk_log2, uint64_t* min_nnz);
+int fiber_rule();
 
 // cpd.hip: per-phase event timing of a session's eager iterations (default on)
 int cpd_session_set_timing(alto_cpd_session_t* sess, int on);

@@ -757,6 +757,75 @@ __global__ void __launch_bounds__(256) k_mode_gather(const uint64_t* __restrict_
     }
 }
 
+// ------------------------------------------------------------------ fiber layout
+// Sort key of a fiber-ordered stream: (chunk, row of `mode`, row of `amode`).
+template <int W, int PW>
+__global__ void __launch_bounds__(256) k_fiber_key(const uint64_t* __restrict__ keys, LutRef g_lut, PackLayout L,
+                                                   int mode, int amode, int chunk_log2, uint64_t n,
+                                                   uint64_t* __restrict__ ck, uint32_t* __restrict__ pay)
+{
+    extern __shared__ uint64_t s_lut[];
+    const uint64_t* lut = stage_lut<PW>(s_lut, g_lut);
+    const int ba = L.bits[amode], bn = L.bits[mode];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t k[W], pk[PW];
+#pragma unroll
+        for (int w = 0; w < W; ++w) k[w] = keys[i * W + w];
+        lut_apply<W, PW>(lut, g_lut.nb, k, pk);
+        const uint64_t r = field<PW>(pk, L.off[mode], bn), a = field<PW>(pk, L.off[amode], ba);
+        ck[i] = (((i >> chunk_log2) << bn | r) << ba) | a;
+        pay[i] = (uint32_t)i;
+    }
+}
+
+// h[i] = 1 where a fiber starts in the sorted keys
+__global__ void k_fiber_heads(const uint64_t* __restrict__ ck, uint64_t n, uint32_t* __restrict__ h)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        h[i] = (i == 0 || ck[i] != ck[i - 1]) ? 1u : 0u;
+}
+
+// id: in, the exclusive scan of the heads; out, the fiber of each nonzero.
+// fstart[f] = first position of fiber f, fstart[nfibers] = n.
+__global__ void k_fiber_starts(const uint64_t* __restrict__ ck, uint64_t n, uint32_t* __restrict__ id,
+                               uint32_t* __restrict__ fstart)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t head = (i == 0 || ck[i] != ck[i - 1]) ? 1u : 0u;
+        const uint32_t f = id[i] + head - 1;
+        id[i] = f;
+        if (head) fstart[f] = (uint32_t)i;
+        if (i == n - 1) fstart[f + 1] = (uint32_t)n;
+    }
+}
+
+__device__ inline uint32_t fiber_long(const uint32_t* id, const uint32_t* fstart, uint64_t i, uint32_t min_len)
+{
+    const uint32_t f = id[i];
+    return fstart[f + 1] - fstart[f] >= min_len ? 1u : 0u;
+}
+
+__global__ void k_fiber_class(const uint32_t* __restrict__ id, const uint32_t* __restrict__ fstart, uint64_t n,
+                              uint32_t min_len, uint32_t* __restrict__ cls)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        cls[i] = fiber_long(id, fstart, i, min_len);
+}
+
+// stable split: long-fiber nonzeros to [0, nlong), the rest after them;
+// pos = exclusive scan of the long flags
+__global__ void k_fiber_scatter(const uint32_t* __restrict__ id, const uint32_t* __restrict__ fstart,
+                                const uint32_t* __restrict__ pos, const uint32_t* __restrict__ pay, uint64_t n,
+                                uint32_t min_len, uint32_t* __restrict__ perm, uint64_t* __restrict__ nlong_out)
+{
+    const uint64_t nlong = (uint64_t)pos[n - 1] + fiber_long(id, fstart, n - 1, min_len);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t p = fiber_long(id, fstart, i, min_len) ? pos[i] : nlong + (i - pos[i]);
+        perm[p] = pay[i];
+        if (i == 0) *nlong_out = nlong;
+    }
+}
+
 #define DISPATCH_WPW(W_, PW_, ...)                                              \
     do {                                                                        \
         if ((W_) == 1 && (PW_) == 1) { constexpr int W = 1, PW = 1; __VA_ARGS__; } \
@@ -1022,19 +1091,85 @@ static bool rank_applicable(const alto_tensor* t)
     return true;
 }
 
+// the order a mode's stream should have with the current settings
+struct StreamLayout {
+    int chunk_log2, fib_mode, fib_min;
+};
+static StreamLayout wanted_layout(const alto_tensor* t, int m, int ranked)
+{
+    StreamLayout L{t->chunk_log2, -1, 0};
+    int fmin = 0, fchunk = 0;
+    uint64_t fnnz = 0;
+    if (ranked || !fiber_settings(&fmin, &fchunk, &fnnz) || t->nmode < 3 || t->nnz < fnnz) return L;
+    int a = -1;  // the fiber mode: the other mode with the fewest (rule 1: most) rows
+    const int most = fiber_rule();
+    for (int q = 0; q < t->nmode; ++q)
+        if (q != m && (a < 0 || (most ? t->dims[q] > t->dims[a] : t->dims[q] < t->dims[a]))) a = q;
+    const uint64_t nch = (t->nnz + (1ull << fchunk) - 1) >> fchunk;
+    int cb = 0;
+    while ((1ull << cb) < nch) ++cb;
+    if (cb + t->pack.bits[m] + t->pack.bits[a] > 64) return L;
+    L.chunk_log2 = fchunk;
+    L.fib_mode = a;
+    L.fib_min = fmin;
+    return L;
+}
+
+// Orders the nonzeros of stream m by (chunk, row_m, row_a) and splits them
+// into long- and short-fiber parts; the permutation ends in `pay`, the
+// length of the first part in *nlong.  Synchronous on `s`.
+static int fiber_order(alto_tensor* t, int m, const StreamLayout& L, DevBuf& ck, DevBuf& pay, hipStream_t s,
+                       uint64_t* nlong)
+{
+    const uint64_t n = t->nnz;
+    const uint64_t nch = (n + (1ull << L.chunk_log2) - 1) >> L.chunk_log2;
+    int cb = 0;
+    while ((1ull << cb) < nch) ++cb;
+    DISPATCH_WPW(t->W, t->pack.pw, {
+        hipLaunchKernelGGL((k_fiber_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
+                           t->key.as<uint64_t>(), lutref(t->delin), t->pack, m, L.fib_mode, L.chunk_log2, n,
+                           ck.as<uint64_t>(), pay.as<uint32_t>());
+    });
+    HIP_TRY(hipGetLastError());
+    ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n,
+                              cb + t->pack.bits[m] + t->pack.bits[L.fib_mode], s));
+    DevBuf id, fstart, cls, perm, tmp, nl;
+    ALTO_TRY(id.alloc(n * 4));
+    ALTO_TRY(fstart.alloc((n + 1) * 4));
+    ALTO_TRY(cls.alloc(n * 4));
+    ALTO_TRY(perm.alloc(n * 4));
+    ALTO_TRY(nl.alloc(8));
+    hipLaunchKernelGGL(k_fiber_heads, dim3(grid_for(n)), dim3(256), 0, s, ck.as<uint64_t>(), n, id.as<uint32_t>());
+    ALTO_TRY(scan_exclusive(id.as<uint32_t>(), n, tmp, s));
+    hipLaunchKernelGGL(k_fiber_starts, dim3(grid_for(n)), dim3(256), 0, s, ck.as<uint64_t>(), n, id.as<uint32_t>(),
+                       fstart.as<uint32_t>());
+    hipLaunchKernelGGL(k_fiber_class, dim3(grid_for(n)), dim3(256), 0, s, id.as<uint32_t>(), fstart.as<uint32_t>(), n,
+                       (uint32_t)L.fib_min, cls.as<uint32_t>());
+    ALTO_TRY(scan_exclusive(cls.as<uint32_t>(), n, tmp, s));
+    hipLaunchKernelGGL(k_fiber_scatter, dim3(grid_for(n)), dim3(256), 0, s, id.as<uint32_t>(), fstart.as<uint32_t>(),
+                       cls.as<uint32_t>(), pay.as<uint32_t>(), n, (uint32_t)L.fib_min, perm.as<uint32_t>(),
+                       nl.as<uint64_t>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(nlong, nl.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(pay.p, perm.p, n * 4, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ALTO_OK;
+}
+
 int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
 {
     uint64_t n = t->nnz;
     const int want_chunk = tuned_chunk_log2();
     if (want_chunk > 0 && want_chunk != t->chunk_log2) t->chunk_log2 = want_chunk;
-    uint64_t nchunks = (n + (1ull << t->chunk_log2) - 1) >> t->chunk_log2;
-    int cbits = 0;
-    while ((1ull << cbits) < nchunks) ++cbits;
     const int ranked = (int)rank_applicable(t);
+    StreamLayout want[ALTO_MAXN];
     bool any = false, reorder = false;
     for (int m = 0; m < t->nmode; ++m) {
         const ModeStream& M = t->ms[m];
-        if (M.chunk_log2 != t->chunk_log2 || M.ranked != ranked) reorder = true;
+        want[m] = wanted_layout(t, m, ranked);
+        if (M.chunk_log2 != want[m].chunk_log2 || M.ranked != ranked || M.fib_mode != want[m].fib_mode ||
+            M.fib_min != want[m].fib_min)
+            reorder = true;
         if ((need64 && !M.val64.p) || (need32 && !M.val32.p) || !M.pk.p) any = true;
     }
     if (!any && !reorder) return ALTO_OK;
@@ -1043,13 +1178,17 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     ALTO_TRY(t->wait_uses());
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
-        if (M.chunk_log2 != t->chunk_log2 || M.ranked != ranked) {
+        if (M.chunk_log2 != want[m].chunk_log2 || M.ranked != ranked || M.fib_mode != want[m].fib_mode ||
+            M.fib_min != want[m].fib_min) {
             M.pk.release();
             M.val64.release();
             M.val32.release();
             M.ranked = 0;
             M.split = 0;
             M.pat_start.clear();
+            M.fib_mode = -1;
+            M.fib_min = 0;
+            M.nlong = 0;
         }
     }
     DevBuf ck, pay;
@@ -1058,7 +1197,7 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
         if (M.pk.p && (!need64 || M.val64.p) && (!need32 || M.val32.p)) continue;
-        M.chunk_log2 = t->chunk_log2;
+        M.chunk_log2 = want[m].chunk_log2;
         M.ranked = 0;
         M.split = 0;
         M.pat_start.clear();
@@ -1068,13 +1207,27 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         if (!M.pk.p) ALTO_TRY(M.pk.alloc(n * t->pack.pw * 8));
         if (w64 && !M.val64.p) ALTO_TRY(M.val64.alloc(n * 8));
         if (w32 && !M.val32.p) ALTO_TRY(M.val32.alloc(n * 4));
-        DISPATCH_WPW(t->W, t->pack.pw, {
-            hipLaunchKernelGGL((k_mode_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s, t->key.as<uint64_t>(),
-                               lutref(t->delin), t->pack, m, t->chunk_log2, n, ck.as<uint64_t>(),
-                               pay.as<uint32_t>());
-        });
-        HIP_TRY(hipGetLastError());
-        ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, t->pack.bits[m] + cbits, s));
+        if (want[m].fib_mode >= 0) {
+            uint64_t nlong = 0;
+            ALTO_TRY(fiber_order(t, m, want[m], ck, pay, s, &nlong));
+            M.fib_mode = want[m].fib_mode;
+            M.fib_min = want[m].fib_min;
+            M.nlong = nlong;
+        } else {
+            uint64_t nchunks = (n + (1ull << M.chunk_log2) - 1) >> M.chunk_log2;
+            int cbits = 0;
+            while ((1ull << cbits) < nchunks) ++cbits;
+            DISPATCH_WPW(t->W, t->pack.pw, {
+                hipLaunchKernelGGL((k_mode_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
+                                   t->key.as<uint64_t>(), lutref(t->delin), t->pack, m, M.chunk_log2, n,
+                                   ck.as<uint64_t>(), pay.as<uint32_t>());
+            });
+            HIP_TRY(hipGetLastError());
+            ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, t->pack.bits[m] + cbits, s));
+            M.fib_mode = -1;
+            M.fib_min = 0;
+            M.nlong = 0;
+        }
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_gather<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
                                t->key.as<uint64_t>(), t->vals.as<double>(), lutref(t->delin),

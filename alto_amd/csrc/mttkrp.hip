@@ -56,11 +56,27 @@ static int g_graphs = 0;
 static int g_hot = 0;
 static int g_hot_block = 512;
 static int g_hot_lds = 32768;
-static int g_narrow_max = 16;
+static int g_narrow_max = 32;
 static int g_narrow_small = 1;
 // Phi kernel: 0 = runtime batch loop with LDS permutes, 1 = compile-time
 // batch with DPP broadcasts (5% faster on the 120M-count CP-APR workload)
 static int g_phi_variant = 1;
+// Fiber layout of the mode streams (tensors from 4M nonzeros, 3 or more
+// modes): fibers of at least g_fiber_min nonzeros are walked with the
+// fiber-mode factor row held in registers.
+static int g_fibers = 1;
+static int g_fiber_min = 4;
+static int g_fiber_chunk = 30;
+static uint64_t g_fiber_min_nnz = 4ull << 20;
+static int g_fiber_rule = 0;  // fiber mode: 0 = the other mode with the fewest rows, 1 = the most
+int fiber_rule() { return g_fiber_rule; }
+int fiber_settings(int* min_len, int* chunk_log2, uint64_t* min_nnz)
+{
+    *min_len = g_fiber_min;
+    *chunk_log2 = g_fiber_chunk;
+    *min_nnz = g_fiber_min_nnz;
+    return g_fibers;
+}
 int graphs_enabled() { return g_graphs; }
 int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
@@ -419,7 +435,7 @@ int alto_tune_set(int param, int value)
     if (param == ALTO_TUNE_SEG) return set_seg(value);
     if (param == ALTO_TUNE_CHUNK_LOG2) return set_chunk(value);
     if (param == ALTO_TUNE_NARROW_RANK) {
-        if (value < 0 || value > 16) return ALTO_ERR_ARG;
+        if (value < 0 || value > 128) return ALTO_ERR_ARG;
         g_narrow_max = value;
         return ALTO_OK;
     }
@@ -453,6 +469,30 @@ int alto_tune_set(int param, int value)
     if (param == ALTO_TUNE_PHI_VARIANT) {
         if (value < 0 || value > 1) return ALTO_ERR_ARG;
         g_phi_variant = value;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_FIBERS) {
+        g_fibers = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_FIBER_MIN) {
+        if (value < 2 || value > (1 << 20)) return ALTO_ERR_ARG;
+        g_fiber_min = value;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_FIBER_CHUNK_LOG2) {
+        if (value < 10 || value > 31) return ALTO_ERR_ARG;
+        g_fiber_chunk = value;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_FIBER_MODE_RULE) {
+        if (value < 0 || value > 1) return ALTO_ERR_ARG;
+        g_fiber_rule = value;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_FIBER_MIN_NNZ) {
+        if (value < 0) return ALTO_ERR_ARG;
+        g_fiber_min_nnz = (uint64_t)value;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_NONTEMPORAL) {

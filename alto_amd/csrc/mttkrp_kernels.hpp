@@ -21,7 +21,7 @@ struct MttkrpTune {
     int nt = 0;             // non-temporal stream loads
     int hot_block = 512;    // threads per workgroup of the ranked-stream kernel
     int hot_lds = 49152;    // LDS bytes per workgroup for hot rows
-    int narrow_max = 16;    // FP32 / BF16 ranks that use two columns per lane
+    int narrow_max = 32;    // FP32 / BF16 ranks that use two columns per lane
     int narrow_small = 1;   // ... and every rank below 4M nonzeros
 };
 
@@ -465,6 +465,165 @@ __global__ void __launch_bounds__(256) k_mttkrp16(StreamArgs<FT, VT, OT> a)
     mttkrp16_body<FT, VT, AT, OT, G, E, PW, NO, HM, false>(a, block, hot_smem);
 }
 
+// MTTKRP over a fiber-ordered stream (nonzeros ordered by (row, row of the
+// fiber mode a)): as k_mttkrp16, but the fiber-mode row U_a[row_a] is read
+// only where a fiber starts.  In each block of J nonzeros of a group, a
+// position whose fiber differs from the previous position's (for the first
+// position: from the previous block's last) is a fiber head; the wave loads
+// U_a rows at the positions where any of its groups has a head -- a
+// wave-uniform branch, so every lane loads (groups without a head reload
+// the row they already hold) and nothing is merged into live registers
+// between the issues -- and every other position reuses the row of the
+// last loaded one.  On long fibers most blocks need no U_a load: a
+// 3-mode MTTKRP then gathers one row per nonzero instead of two.  The other
+// NB = N - 2 modes' rows are gathered per nonzero; products are summed per
+// output row and flushed as in k_mttkrp16.
+template <typename FT, typename VT, typename AT, typename OT, int G, int E, int PW, int NB>
+__global__ void __launch_bounds__(256) k_mttkrp_fib(StreamArgs<FT, VT, OT> a, const FT* __restrict__ Ua, int offa,
+                                                   int bitsa)
+{
+    constexpr int NS = G < 4 ? 4 / G : 1;
+    constexpr int B = G * NS;
+    constexpr int J = 4;
+    const uint64_t block = xcd_block(a.nblocks_real);
+    if (block == ~0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int gl = lane & (G - 1);
+    const uint64_t group = (block * 256 + threadIdx.x) / G;
+    const uint64_t s = group * a.seg;
+    if (s >= a.nnz) return;
+    const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
+    const int R = a.rank;
+
+    AT acc[E], ua_last[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) acc[q] = ua_last[q] = 0;
+    uint32_t cur = 0xffffffffu;
+    uint32_t prev_a = 0;  // fiber row offset of the previous position
+    bool have_a = false;  // ua_last holds U_a at prev_a
+
+    // this lane's nonzeros of the next batch (past the segment: row 0 of
+    // every mode, value 0 -- valid memory, and never accumulated)
+    uint64_t kn[NS][PW];
+    VT vn[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const uint64_t i = s + k * G + gl;
+        vn[k] = 0;
+#pragma unroll
+        for (int w = 0; w < PW; ++w) kn[k][w] = 0;
+        if (i < e) {
+#pragma unroll
+            for (int w = 0; w < PW; ++w) kn[k][w] = a.pk[i * PW + w];
+            vn[k] = a.val[i];
+        }
+    }
+
+    for (uint64_t p = s; p < e; p += B) {
+        uint32_t offs[NS][NB];
+        uint32_t aoff[NS], rown[NS];
+        VT myv[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            myv[k] = vn[k];
+#pragma unroll
+            for (int o = 0; o < NB; ++o) offs[k][o] = (uint32_t)(field<PW>(kn[k], a.offo[o], a.bitso[o]) * (uint64_t)R);
+            aoff[k] = (uint32_t)(field<PW>(kn[k], offa, bitsa) * (uint64_t)R);
+            rown[k] = (uint32_t)field<PW>(kn[k], a.offn, a.bitsn);
+        }
+        const int cnt = (int)(e - p < (uint64_t)B ? e - p : (uint64_t)B);
+        static_for<0, B / J>([&](auto bi) {
+            constexpr int j0 = decltype(bi)::value * J;
+            if (j0 >= cnt) return;
+            AT prod[J][E];
+            uint32_t rj[J], aj[J];
+            static_for<0, J>([&](auto ji) {
+                constexpr int jj = decltype(ji)::value;
+                constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
+                rj[jj] = bcast32<G, src>(rown[slot]);
+                aj[jj] = bcast32<G, src>(aoff[slot]);
+                const AT v = (AT)bcast_val<G, src>(myv[slot]);
+#pragma unroll
+                for (int q = 0; q < E; ++q) prod[jj][q] = v;
+            });
+            // issue: the other modes' rows of the J nonzeros, then U_a rows
+            // at the fiber heads
+            static_for<0, NB>([&](auto oi) {
+                constexpr int o = decltype(oi)::value;
+                const FT* __restrict__ U = a.Uo[o];
+                static_for<0, J>([&](auto ji) {
+                    constexpr int jj = decltype(ji)::value;
+                    constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
+                    const uint32_t off = bcast32<G, src>(offs[slot][o]);
+                    AT x[E];
+                    load16<FT, AT, E>(U + off, gl, R, x);
+#pragma unroll
+                    for (int q = 0; q < E; ++q) prod[jj][q] *= x[q];
+                });
+            });
+            // lanes beyond the rank load the row's first vector and keep it:
+            // they never flush, so no select follows the load
+            AT ra[J][E];
+            uint32_t ldm = 0;  // positions loaded (uniform)
+            const int c0 = gl * E < R ? gl * E : 0;
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const bool head = jj == 0 ? (!have_a || aj[0] != prev_a) : aj[jj] != aj[jj - 1];
+                if (__any(head)) {
+                    VecLoad<FT, E>::load(Ua + aj[jj] + c0, ra[jj]);
+                    ldm |= 1u << jj;
+                }
+            }
+            // the mask reaches the selection below through an opaque copy, so
+            // that the selection is not folded into the branches above (which
+            // would chain every load to the next one's registers)
+            asm volatile("" : "+s"(ldm));
+            if constexpr (j0 == 0) {
+                // the next batch's nonzeros, issued behind this block's gathers
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    const uint64_t i = p + B + k * G + gl;
+                    vn[k] = 0;
+#pragma unroll
+                    for (int w = 0; w < PW; ++w) kn[k][w] = 0;
+                    if (i < e) {
+#pragma unroll
+                        for (int w = 0; w < PW; ++w) kn[k][w] = a.pk[i * PW + w];
+                        vn[k] = a.val[i];
+                    }
+                }
+            }
+            // math: each position's U_a row is that of the last load at or
+            // before it
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const bool use = (ldm >> jj) & 1u;
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    ua_last[q] = use ? ra[jj][q] : ua_last[q];
+                    prod[jj][q] *= ua_last[q];
+                }
+            }
+            prev_a = aj[J - 1];
+            have_a = true;
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                if (j0 + jj < cnt) {
+                    if (rj[jj] != cur) {
+                        if (cur != 0xffffffffu) flush_row<AT, OT, G, E, true>(a.out, cur, R, gl, acc);
+                        cur = rj[jj];
+#pragma unroll
+                        for (int q = 0; q < E; ++q) acc[q] = 0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < E; ++q) acc[q] += prod[jj][q];
+                }
+            }
+        });
+    }
+    if (cur != 0xffffffffu) flush_row<AT, OT, G, E, true>(a.out, cur, R, gl, acc);
+}
+
 // Many MTTKRPs (different tensors, modes, sizes; one rank and lane shape) in
 // one launch: the table travels in the kernel arguments, and job j owns the
 // logical workgroups [start[j], start[j+1]) -- its own launch's grid, laid
@@ -496,6 +655,81 @@ inline bool rows_aligned(const StreamArgs<FT, VT, OT>& a, int R, int bytes)
     for (int o = 0; o < a.nother; ++o)
         if ((uintptr_t)a.Uo[o] % bytes != 0) return false;
     return true;
+}
+
+// The long-fiber part [0, nlong) of a fiber-ordered stream through
+// k_mttkrp_fib (G lanes per nonzero, E columns per lane); `base` carries the
+// stream, value, output and target-mode fields.
+template <typename FT, typename VT, typename AT, typename OT, int E>
+int launch_fiber_part(const alto_tensor* t, int mode, int R, const FT* const* U, const StreamArgs<FT, VT, OT>& base,
+                      int G, int seg, hipStream_t s)
+{
+    const ModeStream& M = t->ms[mode];
+    StreamArgs<FT, VT, OT> f = base;
+    int o = 0;
+    for (int m = 0; m < t->nmode; ++m) {
+        if (m == mode || m == M.fib_mode) continue;
+        f.Uo[o] = U[m];
+        f.offo[o] = t->pack.off[m];
+        f.bitso[o] = t->pack.bits[m];
+        ++o;
+    }
+    for (int q = o; q < ALTO_MAXN - 1; ++q) {
+        f.Uo[q] = nullptr;
+        f.offo[q] = f.bitso[q] = 0;
+    }
+    f.nother = o;
+    f.nnz = M.nlong;
+    f.seg = (uint64_t)seg;
+    const dim3 grid = stream_grid(f.nnz, seg, G, f);
+    const FT* Ua = U[M.fib_mode];
+    const int offa = t->pack.off[M.fib_mode], bitsa = t->pack.bits[M.fib_mode];
+#define KF_(GG, PWW, NBB) \
+    hipLaunchKernelGGL((k_mttkrp_fib<FT, VT, AT, OT, GG, E, PWW, NBB>), grid, dim3(256), 0, s, f, Ua, offa, bitsa)
+#define KFN_(GG, PWW)                          \
+    do {                                       \
+        if (o == 1) KF_(GG, PWW, 1);           \
+        else if (o == 2) KF_(GG, PWW, 2);      \
+        else KF_(GG, PWW, 3);                  \
+    } while (0)
+#define KFG_(GG)                               \
+    do {                                       \
+        if (G == GG) {                         \
+            if (t->pack.pw == 1) KFN_(GG, 1);  \
+            else KFN_(GG, 2);                  \
+        }                                      \
+    } while (0)
+    KFG_(1);
+    KFG_(2);
+    KFG_(4);
+    KFG_(8);
+    KFG_(16);
+    KFG_(32);
+    KFG_(64);
+#undef KFG_
+#undef KFN_
+#undef KF_
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+// launch_fiber_part instantiated per element type and columns per lane in
+// mttkrp_fib_{f64,f32,bf16}.hip (translation units of their own, built in
+// parallel with the others)
+int fiber_part(const alto_tensor* t, int mode, int R, const double* const* U,
+               const StreamArgs<double, double, double>& a, int G, int E, int seg, hipStream_t s);
+int fiber_part(const alto_tensor* t, int mode, int R, const float* const* U, const StreamArgs<float, float, double>& a,
+               int G, int E, int seg, hipStream_t s);
+int fiber_part(const alto_tensor* t, int mode, int R, const bf16_t* const* U,
+               const StreamArgs<bf16_t, float, double>& a, int G, int E, int seg, hipStream_t s);
+
+// the stream arguments narrowed to the short-fiber part [nlong, nnz)
+template <typename FT, typename VT, typename OT>
+inline void skip_long_part(const alto_tensor* t, uint64_t nlong, StreamArgs<FT, VT, OT>& a)
+{
+    a.pk += nlong * t->pack.pw;
+    a.val += nlong;
+    a.nnz -= nlong;
 }
 
 // Lane mappings: 0 = 2 strided columns per lane (each gather instruction
@@ -532,12 +766,18 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     // FP64 there.
     constexpr bool NARROW_OK = !std::is_same<FT, double>::value;
     const bool narrow_rank = R <= k.narrow_max || (k.narrow_small && t->nnz < (4ull << 20) && R <= 128);
+    const bool fib = M.fib_mode >= 0 && M.nlong > 0 && !hot && a.nother >= 2;
     if (v == 2 && NARROW_OK && narrow_rank && !split && a.nother >= 1 && rows_aligned(a, R, 2 * (int)sizeof(FT))) {
         int G = 1;
         while (G * 2 < R) G <<= 1;
         if (seg <= 0) seg = tuned_seg(t->nnz, G, (int)sizeof(AT));
         a.seg = (uint64_t)seg;
-        dim3 grid = stream_grid(t->nnz, seg, G, a);
+        if (fib) {
+            ALTO_TRY(fiber_part(t, mode, R, U, a, G, 2, seg, s));
+            skip_long_part(t, M.nlong, a);
+            if (a.nnz == 0) return ALTO_OK;
+        }
+        dim3 grid = stream_grid(a.nnz, seg, G, a);
 #define KN_(GG, PWW)                                                                                            \
     do {                                                                                                        \
         switch (a.nother) {                                                                                     \
@@ -571,6 +811,11 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
         while (G * E16 < R) G <<= 1;
         if (seg <= 0) seg = tuned_seg(t->nnz, G, (int)sizeof(AT));
         a.seg = (uint64_t)seg;
+        if (fib && !split) {
+            ALTO_TRY(fiber_part(t, mode, R, U, a, G, E16, seg, s));
+            skip_long_part(t, M.nlong, a);
+            if (a.nnz == 0) return ALTO_OK;
+        }
         // one launch per hot-pattern group (one launch over the whole stream
         // when it is not split)
         const int npat = split ? 1 << a.nother : 1;

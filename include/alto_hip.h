@@ -145,15 +145,27 @@ int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* mod
  * (256/512/1024).
  * ALTO_TUNE_HOT_LDS: LDS bytes per workgroup for the staged rows (default
  * 32768).
- * ALTO_TUNE_NARROW_RANK: FP32 / BF16 ranks up to this value (0..16, default
- * 16) use two columns per lane in the vector-lane mapping.
+ * ALTO_TUNE_NARROW_RANK: FP32 / BF16 ranks up to this value (0..128,
+ * default 32) use two columns per lane in the vector-lane mapping (fewer
+ * lane groups per wave, so fewer rounds of FP64 row flushes).
  * ALTO_TUNE_PHI_VARIANT: CP-APR Phi kernel, 0 = runtime batch loop with LDS
  * permutes, 1 = compile-time batch with DPP broadcasts (default).
  * ALTO_TUNE_NARROW_SMALL: 1 (default) = below 4M nonzeros every FP32 / BF16
- * rank uses two columns per lane. */
+ * rank uses two columns per lane.
+ * ALTO_TUNE_FIBERS: 1 (default) = mode streams of tensors with at least
+ * ALTO_TUNE_FIBER_MIN_NNZ nonzeros (default 4M) and 3+ modes are ordered by
+ * (row, row of a second "fiber" mode -- the other mode with the fewest
+ * rows) inside chunks of 2^ALTO_TUNE_FIBER_CHUNK_LOG2 nonzeros (default 30)
+ * and split: the nonzeros of fibers of at least ALTO_TUNE_FIBER_MIN
+ * (default 4) nonzeros first, walked by a kernel that reads the fiber-mode
+ * factor row once per fiber, then the rest.  ALTO_TUNE_FIBER_MODE_RULE: 0 =
+ * the fiber mode is the other mode with the fewest rows, 1 = with the most.
+ * Streams are re-ordered on their next use. */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
        ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9,
-       ALTO_TUNE_NARROW_RANK = 10, ALTO_TUNE_PHI_VARIANT = 11, ALTO_TUNE_NARROW_SMALL = 12 };
+       ALTO_TUNE_NARROW_RANK = 10, ALTO_TUNE_PHI_VARIANT = 11, ALTO_TUNE_NARROW_SMALL = 12,
+       ALTO_TUNE_FIBERS = 13, ALTO_TUNE_FIBER_MIN = 14, ALTO_TUNE_FIBER_CHUNK_LOG2 = 15,
+       ALTO_TUNE_FIBER_MIN_NNZ = 16, ALTO_TUNE_FIBER_MODE_RULE = 17 };
 int alto_tune_set(int param, int value);
 
 /* Autotuner (no reference counterpart; the reference fixes its shapes at
