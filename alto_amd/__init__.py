@@ -23,6 +23,8 @@ VAL_F64, VAL_U64 = 0, 1
 PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
 PTR_HOST, PTR_DEVICE = 0, 1
 MAX_MODES = 5
+APR_STOP_ALTO, APR_BENCH, APR_STOP_COO = 0, 1, 2
+KRUSKAL_L1, KRUSKAL_NORM2, KRUSKAL_NORMMAX, KRUSKAL_REDISTRIBUTE = 0, 1, 2, 3
 
 _lib = None
 
@@ -113,6 +115,11 @@ def lib():
         "alto_cpd_als": (i32, [vp, u64, i32, f64, i32, vp, vp, vp, C.POINTER(i32), C.POINTER(CpdTimers), i32, vp]),
         "alto_cpd_fit": (i32, [vp, u64, vp, vp, C.POINTER(f64)]),
         "alto_cp_apr": (i32, [vp, u64, i32, i32, f64, f64, f64, f64, i32, vp, vp, C.POINTER(AprResult), i32, vp]),
+        "alto_cp_apr_ex": (i32, [vp, u64, i32, i32, f64, f64, f64, f64, i32, vp, vp, C.POINTER(AprResult), i32,
+                                 vp]),
+        "alto_cpd_fit_parts": (i32, [i32, vp, u64, vp, vp, vp, vp, f64, i32, C.POINTER(f64)]),
+        "alto_kruskal_normalize": (i32, [i32, vp, u64, vp, vp, i32, i32, i32, vp]),
+        "alto_coo_mttkrp": (i32, [i32, vp, u64, vp, vp, i32, u64, vp, vp, i32, i32, vp]),
         "alto_update_gram": (i32, [vp, u64, u64, i32, vp, vp]),
         "alto_update_gram_noalloc": (i32, [vp, u64, u64, i32, vp, vp]),
         "alto_cpd_session_create": (i32, [vp, u64, i32, vp, vp, C.POINTER(vp)]),
@@ -129,6 +136,7 @@ def lib():
         "alto_device_free": (i32, [vp]),
         "alto_memcpy_h2d": (i32, [vp, vp, u64]),
         "alto_memcpy_d2h": (i32, [vp, vp, u64]),
+        "alto_memcpy_d2d": (i32, [vp, vp, u64, vp]),
         "alto_device_synchronize": (i32, []),
     }
     for name, (res, args) in sig.items():
@@ -345,15 +353,19 @@ class AltoTensor:
         return iters.value, fit[: iters.value], {k: getattr(tm, k) for k, _ in CpdTimers._fields_}
 
     def cp_apr(self, factors, lam, max_iters, max_inner, epsilon=1e-5, eps_div_zero=1e-10, kappa_tol=1e-10,
-               kappa=1e-2, dobench=False, verbose=False):
+               kappa=1e-2, dobench=False, verbose=False, stop_rule=None):
+        """cp_apr_alto (stop_rule None: APR_BENCH when dobench, else
+        APR_STOP_ALTO) or the COO drivers' rule (APR_STOP_COO)."""
         fs = [np.ascontiguousarray(f, dtype=np.float64) for f in factors]
         for i, f in enumerate(fs):
             if f is not factors[i]:
                 raise ValueError("factors must be C-contiguous float64 arrays")
         res = AprResult()
-        check(lib().alto_cp_apr(self.h, fs[0].shape[1], max_iters, max_inner, epsilon, eps_div_zero, kappa_tol, kappa,
-                                int(dobench), _ptr_array([f.ctypes.data for f in fs]), lam.ctypes.data,
-                                C.byref(res), int(verbose), None), "alto_cp_apr")
+        if stop_rule is None:
+            stop_rule = APR_BENCH if dobench else APR_STOP_ALTO
+        check(lib().alto_cp_apr_ex(self.h, fs[0].shape[1], max_iters, max_inner, epsilon, eps_div_zero, kappa_tol,
+                                   kappa, int(stop_rule), _ptr_array([f.ctypes.data for f in fs]), lam.ctypes.data,
+                                   C.byref(res), int(verbose), None), "alto_cp_apr_ex")
         return dict(loglik=res.loglik, ls_fit=res.ls_fit, kkt=res.kkt, outer_iters=res.outer_iters,
                     total_inner=res.total_inner, inner_per_mode=list(res.inner_per_mode),
                     t_pre=res.t_pre, t_apr=res.t_apr, t_phi=res.t_phi, t_post=res.t_post)
@@ -393,6 +405,53 @@ class CpdSession:
         if h and _lib is not None:
             _lib.alto_cpd_session_destroy(h)
             self.h = None
+
+
+def coo_mttkrp(dims, cidx, vals, mode, factors, out=None):
+    """alto_coo_mttkrp on host arrays: out (dims[mode] x R) += the mode-`mode`
+    MTTKRP of the coordinate tensor (cidx: nmodes x nnz uint64); a new zero
+    output when out is None.  Returns out."""
+    cidx = np.ascontiguousarray(cidx, np.uint64)
+    vals = np.ascontiguousarray(vals, np.float64)
+    fs = [np.ascontiguousarray(f, np.float64) for f in factors]
+    R = fs[0].shape[1]
+    acc = out is not None
+    if out is None:
+        out = np.zeros((dims[mode], R), np.float64)
+    if not (out.flags.c_contiguous and out.dtype == np.float64):
+        raise ValueError("out must be a C-contiguous float64 array")
+    d = (C.c_uint64 * len(dims))(*dims)
+    check(lib().alto_coo_mttkrp(len(dims), d, vals.size, _ptr_array([cidx[m].ctypes.data for m in range(len(dims))]),
+                                vals.ctypes.data, mode, R, _ptr_array([f.ctypes.data for f in fs]),
+                                out.ctypes.data, int(acc), PTR_HOST, None), "alto_coo_mttkrp")
+    return out
+
+
+def kruskal_normalize(factors, lam, mode, kind):
+    """alto_kruskal_normalize on host arrays (updated in place): KRUSKAL_L1
+    (mode -1: every mode), KRUSKAL_NORM2, KRUSKAL_NORMMAX, KRUSKAL_REDISTRIBUTE."""
+    for f in list(factors) + [lam]:
+        if not (f.flags.c_contiguous and f.dtype == np.float64):
+            raise ValueError("factors and lambda must be C-contiguous float64 arrays")
+    dims = (C.c_uint64 * len(factors))(*[f.shape[0] for f in factors])
+    check(lib().alto_kruskal_normalize(len(factors), dims, factors[0].shape[1],
+                                       _ptr_array([f.ctypes.data for f in factors]), lam.ctypes.data, mode, kind,
+                                       PTR_HOST, None), "alto_kruskal_normalize")
+
+
+def cpd_fit_parts(grams, lam, U_last, U_mttkrp, normX, dims):
+    """alto_cpd_fit_parts on host arrays: the fit of cpd_fit_alto from its
+    arguments (Grams, lambda, last factor, last-mode MTTKRP, ||X||^2)."""
+    gs = [np.ascontiguousarray(g, np.float64) for g in grams]
+    lam = np.ascontiguousarray(lam, np.float64)
+    U_last = np.ascontiguousarray(U_last, np.float64)
+    U_mttkrp = np.ascontiguousarray(U_mttkrp, np.float64)
+    d = (C.c_uint64 * len(dims))(*dims)
+    fit = C.c_double(0)
+    check(lib().alto_cpd_fit_parts(len(dims), d, lam.size, _ptr_array([g.ctypes.data for g in gs]), lam.ctypes.data,
+                                   U_last.ctypes.data, U_mttkrp.ctypes.data, float(normX), PTR_HOST,
+                                   C.byref(fit)), "alto_cpd_fit_parts")
+    return fit.value
 
 
 def cpd_sessions_iterate(sessions, n, epsilon=0.0, stream=None):

@@ -15,11 +15,11 @@
 #include <string>
 #include <vector>
 
-#include <thread>
-
 #include "alto/alto.hpp"
-#include "poisson_gen.hpp"
-#include "tensor_io.hpp"
+#include "alto/apr.hpp"
+#include "alto/cpd.hpp"
+#include "alto/poisson_generator.hpp"
+#include "alto/sptensor.hpp"
 
 static const char* short_opt = "hvi:o:b:l:r:m:n:x:d:t:s:e:f:zcp";
 static const struct option long_opt[] = {
@@ -74,19 +74,6 @@ static void print_info(IType rank, int max_iters, SparseTensor* X)
     fprintf(stdout, "NNZ             = %llu\n", X->nnz);
 }
 
-// sequential COO MTTKRP (include/cpd.hpp:509-543): the truth of the -c check
-static void coo_mttkrp(SparseTensor* X, FType** U, IType rank, int mode, FType* out)
-{
-    std::vector<FType> row(rank);
-    for (IType i = 0; i < X->nnz; ++i) {
-        for (IType r = 0; r < rank; ++r) row[r] = X->vals[i];
-        for (int m = 0; m < X->nmodes; ++m)
-            if (m != mode)
-                for (IType r = 0; r < rank; ++r) row[r] *= U[m][X->cidx[m][i] * rank + r];
-        for (IType r = 0; r < rank; ++r) out[X->cidx[mode][i] * rank + r] += row[r];
-    }
-}
-
 template <typename LIT>
 static int run(SparseTensor* X, Model model, IType rank, int max_iters, int max_inner, double epsilon, int seed,
                int target_mode, bool do_check, bool do_bench, int nprtn)
@@ -100,11 +87,14 @@ static int run(SparseTensor* X, Model model, IType rank, int max_iters, int max_
         KruskalModelRandomInit(M, (unsigned)seed);
         AltoTensor<LIT>* AT;
         create_alto(X, &AT, nprtn);
+        // the truth: the COO MTTKRP (include/cpd.hpp:509-543), straight from
+        // the coordinates; then the ALTO MTTKRP from the same factors
         for (int m = 0; m < X->nmodes; ++m) {
             if (target_mode != -1 && m != target_mode) continue;
-            std::vector<FType> truth(X->dims[m] * rank, 0.0), saved(M->U[m], M->U[m] + X->dims[m] * rank);
-            coo_mttkrp(X, M->U, rank, m, truth.data());
-            memset(M->U[m], 0, X->dims[m] * rank * sizeof(FType));
+            std::vector<FType> saved(M->U[m], M->U[m] + X->dims[m] * rank);
+            mttkrp(X, M, (IType)m);
+            std::vector<FType> truth(M->U[m], M->U[m] + X->dims[m] * rank);
+            memcpy(M->U[m], saved.data(), saved.size() * sizeof(FType));
             mttkrp_alto_par(m, M->U, rank, AT, nullptr, nullptr);
             IType cnt = 0;
             for (IType k = 0; k < X->dims[m] * rank; ++k)
@@ -199,11 +189,11 @@ int main(int argc, char** argv)
     (void)text_out;
     SparseTensor* X = nullptr;
     if (!bin_file.empty()) {
-        X = read_bin(bin_file.c_str());
-        if (save) write_tensor(export_name(X, false).c_str(), X, false);
+        ImportSparseTensor(bin_file.c_str(), BINARY_FORMAT, &X, zero_based);
+        if (save) ExportSparseTensor(NULL, TEXT_FORMAT, X);
     } else if (!text_file.empty()) {
-        X = read_tns(text_file.c_str(), zero_based);
-        if (save) write_tensor(export_name(X, true).c_str(), X, true);
+        ImportSparseTensor(text_file.c_str(), TEXT_FORMAT, &X, zero_based);
+        if (save) ExportSparseTensor(NULL, BINARY_FORMAT, X);
     } else if (dims_arg.empty()) {
         fprintf(stderr, "No dims specified... exiting\n");
         usage(argv[0]);
@@ -238,21 +228,19 @@ int main(int argc, char** argv)
             fprintf(stderr, "Invalid -sparsity: no nonzeros for %s.\n", dims_arg.c_str());
             return -1;
         }
-        int threads = (int)std::thread::hardware_concurrency();
-        if (const char* e = getenv("OMP_NUM_THREADS")) threads = atoi(e);
-        if (threads < 1) threads = 1;
-        X = alto_gen::poisson_tensor((int)dims.size(), dims.data(), edges, rank, threads);
+        X = alto_gen::poisson_tensor((int)dims.size(), dims.data(), edges, rank, alto_gen::default_threads());
         if (save) {
-            write_tensor(export_name(X, false).c_str(), X, false);
-            write_tensor(export_name(X, true).c_str(), X, true);
+            ExportSparseTensor(NULL, TEXT_FORMAT, X);
+            ExportSparseTensor(NULL, BINARY_FORMAT, X);
         }
     }
     const char* base = strrchr(argv[0], '/');
     base = base ? base + 1 : argv[0];
     const int nprtn = 16;  // reference P = thread count; fixed here for reproducible partitions
-    if (strstr(base, "128"))
-        return run<unsigned __int128>(X, model, rank, max_iters, max_inner, epsilon, seed, target_mode, do_check,
-                                      do_bench, nprtn);
-    return run<unsigned long long>(X, model, rank, max_iters, max_inner, epsilon, seed, target_mode, do_check, do_bench,
-                                   nprtn);
+    const int rc = strstr(base, "128") ? run<unsigned __int128>(X, model, rank, max_iters, max_inner, epsilon, seed,
+                                                                 target_mode, do_check, do_bench, nprtn)
+                                       : run<unsigned long long>(X, model, rank, max_iters, max_inner, epsilon, seed,
+                                                                 target_mode, do_check, do_bench, nprtn);
+    DestroySparseTensor(X);
+    return rc;
 }

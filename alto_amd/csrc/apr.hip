@@ -249,10 +249,22 @@ static int loglik(alto_tensor* t, AprState& c, double eps, double* out, hipStrea
     return normalize_mode(c, t->dims[0], 0, s);
 }
 
+// rule: ALTO_APR_STOP_ALTO -- cp_apr_alto's outer stop, |KKT - previous
+// KKT| <= eps^2 (include/apr.hpp:1261); ALTO_APR_BENCH -- every inner and
+// outer iteration runs (its dobench); ALTO_APR_STOP_COO -- cp_apr_mu_alto's
+// (src/apr.cpp:345-448): an outer iteration in which every mode's first
+// inner step already met the KKT tolerance ends the run, and that
+// iteration is counted.
 int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsilon, double eps_div_zero,
-                double kappa_tol, double kappa, int dobench, double* const* Uh, double* lambda_h, alto_apr_result_t* res,
+                double kappa_tol, double kappa, int rule, double* const* Uh, double* lambda_h, alto_apr_result_t* res,
                 int verbose, hipStream_t s)
 {
+    if (rule != ALTO_APR_STOP_ALTO && rule != ALTO_APR_BENCH && rule != ALTO_APR_STOP_COO) {
+        set_error("cp_apr: unknown stop rule %d", rule);
+        return ALTO_ERR_ARG;
+    }
+    const int dobench = rule == ALTO_APR_BENCH;
+    const bool coo = rule == ALTO_APR_STOP_COO;
     if (R < 1 || R > 128 || max_iters < 1 || max_inner < 1) {
         set_error("cp_apr: rank must be 1..128 and iteration counts >= 1");
         return ALTO_ERR_ARG;
@@ -324,7 +336,7 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
     for (int m = 0; m < N; ++m) Uptr[m] = c.U[m].as<double>();
     int iter;
     for (iter = 0; iter < max_iters; ++iter) {
-        bool converged = false;
+        bool converged = coo;
         for (int n = 0; n < N; ++n) {
             const uint64_t dim = t->dims[n], ne = dim * R;
             double* B = c.U[n].as<double>();
@@ -361,6 +373,7 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
                 }
                 if (dobench) continue;  // B *= Phi already applied
                 if (kv < epsilon) break;
+                converged = false;
                 hipLaunchKernelGGL(k_mu, dim3(nb), dim3(256), 0, s, B, Ph, ne);
                 HIP_TRY(hipGetLastError());
             }
@@ -377,7 +390,7 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
         if (verbose && iter % 10 == 0)
             printf("Iter %d: Inner its = %d KKT violation = %.6e, nViolations = %d\n", iter, n_inner[iter], kkt[iter],
                    n_viol[iter]);
-        if (iter > 0 && fabs(kkt[iter] - kkt[iter - 1]) <= epsilon * epsilon && !dobench) {
+        if (!coo && iter > 0 && fabs(kkt[iter] - kkt[iter - 1]) <= epsilon * epsilon && !dobench) {
             converged = true;
             if (verbose) {
                 printf("Iter %d: Inner its = %d KKT violation = %.6e, nViolations = %d\n", iter, n_inner[iter],
@@ -397,8 +410,11 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
     const double t_apr = now_s() - ta0;
 
     const double tq0 = now_s();
+    // cp_apr_mu_alto counts the iteration it stopped in (src/apr.cpp:415-423)
+    if (coo && iter == max_iters) iter = max_iters - 1;
+    const int counted = coo ? iter + 1 : iter;
     int total_inner = 0;
-    for (int i = 0; i < iter; ++i) total_inner += n_inner[i];
+    for (int i = 0; i < counted; ++i) total_inner += n_inner[i];
     double obj = 0;
     ALTO_TRY(loglik(t, c, eps_div_zero, &obj, s));
     // least-squares fit of the final model
@@ -423,7 +439,7 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
     HIP_TRY(hipStreamSynchronize(s));
     const double t_post = now_s() - tq0;
 
-    const double kkt_last = iter > 0 ? kkt[iter - 1] : 0.0;
+    const double kkt_last = coo ? kkt[iter] : iter > 0 ? kkt[iter - 1] : 0.0;
     if (res) {
         res->loglik = obj;
         res->ls_fit = fb[0];
@@ -452,9 +468,103 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
     return ALTO_OK;
 }
 
+// ------------------------------------------------------------------ Kruskal model utilities
+// per-block column statistics: which 0 = sum of squares, 1 = maximum (signed)
+__global__ void k_colstat_partial(const double* __restrict__ U, uint64_t I, int R, int which, double* __restrict__ part)
+{
+    __shared__ double s[256];
+    const int RP = 256 / R, t = threadIdx.x;
+    double a = which ? -INFINITY : 0.0;
+    if (t < RP * R) {
+        const int c = t % R, ri = t / R;
+        for (uint64_t i = (uint64_t)blockIdx.x * RP + ri; i < I; i += (uint64_t)gridDim.x * RP) {
+            const double v = U[i * R + c];
+            a = which ? fmax(a, v) : a + v * v;
+        }
+    }
+    s[t] = a;
+    __syncthreads();
+    for (int c = t; c < R; c += blockDim.x) {
+        double v = which ? -INFINITY : 0.0;
+        for (int ri = 0; ri < RP; ++ri) v = which ? fmax(v, s[ri * R + c]) : v + s[ri * R + c];
+        part[(size_t)blockIdx.x * R + c] = v;
+    }
+}
+
+// Mat2Norm / MatMaxNorm (src/kruskal_model.cpp:224-305): lambda_r = the
+// column's 2-norm, or max(1, its largest entry) (signed)
+__global__ void k_colstat_final(const double* __restrict__ part, int nb, int R, int which, double* __restrict__ lambda)
+{
+    for (int c = threadIdx.x; c < R; c += blockDim.x) {
+        double a = which ? -INFINITY : 0.0;
+        for (int p = 0; p < nb; ++p) a = which ? fmax(a, part[(size_t)p * R + c]) : a + part[(size_t)p * R + c];
+        lambda[c] = which ? fmax(a, 1.0) : sqrt(a);
+    }
+}
+
+// one Kruskal utility on device factors U (dim x R) and lambda (R)
+static int kruskal_op(double* U, uint64_t dim, int R, double* lambda, int kind, double* part, double* norm,
+                      hipStream_t s)
+{
+    if (kind == ALTO_KRUSKAL_L1) {  // KruskalModelNormalizeMode (src/kruskal_model.cpp:178-222)
+        hipLaunchKernelGGL(k_colabs_partial, dim3(COL_BLOCKS), dim3(256), 0, s, U, dim, R, part);
+        hipLaunchKernelGGL(k_colnorm_final, dim3(1), dim3(128), 0, s, part, COL_BLOCKS, R, norm, lambda);
+        HIP_TRY(hipGetLastError());
+        return scale_device<double, double>(U, dim, R, norm, nullptr, s);
+    }
+    if (kind == ALTO_KRUSKAL_NORM2 || kind == ALTO_KRUSKAL_NORMMAX) {  // KruskalModelNorm (:307-341)
+        const int which = kind == ALTO_KRUSKAL_NORMMAX;
+        hipLaunchKernelGGL(k_colstat_partial, dim3(COL_BLOCKS), dim3(256), 0, s, U, dim, R, which, part);
+        hipLaunchKernelGGL(k_colstat_final, dim3(1), dim3(128), 0, s, part, COL_BLOCKS, R, which, lambda);
+        HIP_TRY(hipGetLastError());
+        return scale_device<double, double>(U, dim, R, lambda, nullptr, s);
+    }
+    // RedistributeLambda (:355-378)
+    hipLaunchKernelGGL(k_mul_cols, dim3(ew_blocks(dim * R)), dim3(256), 0, s, U, dim * R, R, lambda);
+    HIP_TRY(hipGetLastError());
+    return fill_value<double>(lambda, R, 1.0, s);
+}
+
 }  // namespace alto
 
 using namespace alto;
+
+extern "C" int alto_kruskal_normalize(int nmodes, const uint64_t* dims, uint64_t rank, double* const* U, double* lambda,
+                                      int mode, int kind, int ptr_kind, alto_stream_t stream)
+{
+    if (nmodes < 1 || nmodes > ALTO_MAXN || !dims || !U || !lambda || rank < 1 || rank > 128 ||
+        mode < -1 || mode >= nmodes || (mode < 0 && kind != ALTO_KRUSKAL_L1) ||
+        kind < ALTO_KRUSKAL_L1 || kind > ALTO_KRUSKAL_REDISTRIBUTE) {
+        set_error("alto_kruskal_normalize: invalid argument");
+        return ALTO_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int R = (int)rank;
+    const bool host = ptr_kind == ALTO_PTR_HOST;
+    DevBuf part, norm, lam, Ud[ALTO_MAXN];
+    ALTO_TRY(part.alloc((size_t)COL_BLOCKS * R * 8));
+    ALTO_TRY(norm.alloc((size_t)R * 8));
+    double* ld = lambda;
+    if (host) {
+        ALTO_TRY(lam.alloc((size_t)R * 8));
+        HIP_TRY(hipMemcpyAsync(lam.p, lambda, (size_t)R * 8, hipMemcpyHostToDevice, s));
+        ld = lam.as<double>();
+    }
+    const int m0 = mode < 0 ? 0 : mode, m1 = mode < 0 ? nmodes : mode + 1;
+    for (int m = m0; m < m1; ++m) {
+        double* u = U[m];
+        if (host) {
+            ALTO_TRY(Ud[m].alloc(dims[m] * R * 8));
+            HIP_TRY(hipMemcpyAsync(Ud[m].p, U[m], dims[m] * R * 8, hipMemcpyHostToDevice, s));
+            u = Ud[m].as<double>();
+        }
+        ALTO_TRY(kruskal_op(u, dims[m], R, ld, kind, part.as<double>(), norm.as<double>(), s));
+        if (host) HIP_TRY(hipMemcpyAsync(U[m], u, dims[m] * R * 8, hipMemcpyDeviceToHost, s));
+    }
+    if (host) HIP_TRY(hipMemcpyAsync(lambda, ld, (size_t)R * 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));  // temporaries are released below
+    return ALTO_OK;
+}
 
 extern "C" int alto_cp_apr(alto_tensor_t* t, uint64_t rank, int max_iters, int max_inner, double epsilon,
                            double eps_div_zero, double kappa_tol, double kappa, int dobench, double* const* U,
@@ -464,6 +574,18 @@ extern "C" int alto_cp_apr(alto_tensor_t* t, uint64_t rank, int max_iters, int m
         set_error("alto_cp_apr: null argument");
         return ALTO_ERR_ARG;
     }
-    return cp_apr_impl(t, (int)rank, max_iters, max_inner, epsilon, eps_div_zero, kappa_tol, kappa, dobench, U, lambda,
-                       res, verbose, (hipStream_t)stream);
+    return cp_apr_impl(t, (int)rank, max_iters, max_inner, epsilon, eps_div_zero, kappa_tol, kappa,
+                       dobench ? ALTO_APR_BENCH : ALTO_APR_STOP_ALTO, U, lambda, res, verbose, (hipStream_t)stream);
+}
+
+extern "C" int alto_cp_apr_ex(alto_tensor_t* t, uint64_t rank, int max_iters, int max_inner, double epsilon,
+                              double eps_div_zero, double kappa_tol, double kappa, int stop_rule, double* const* U,
+                              double* lambda, alto_apr_result_t* res, int verbose, alto_stream_t stream)
+{
+    if (!t || !U || !lambda) {
+        set_error("alto_cp_apr_ex: null argument");
+        return ALTO_ERR_ARG;
+    }
+    return cp_apr_impl(t, (int)rank, max_iters, max_inner, epsilon, eps_div_zero, kappa_tol, kappa, stop_rule, U,
+                       lambda, res, verbose, (hipStream_t)stream);
 }

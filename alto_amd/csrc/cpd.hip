@@ -775,6 +775,40 @@ int alto_cpd_fit(alto_tensor_t* t, uint64_t rank, double* const* U, const double
     return cpd_fit_impl(t, (int)rank, U, lambda, fit);
 }
 
+int alto_cpd_fit_parts(int nmodes, const uint64_t* dims, uint64_t rank, const double* const* grams,
+                       const double* lambda, const double* U_last, const double* U_mttkrp, double normX, int ptr_kind,
+                       double* fit)
+{
+    if (nmodes < 1 || nmodes > ALTO_MAXN || !dims || !grams || !lambda || !U_last || !U_mttkrp || !fit || rank < 1 ||
+        rank > 128) {
+        set_error("alto_cpd_fit_parts: invalid argument");
+        return ALTO_ERR_ARG;
+    }
+    hipStream_t s = nullptr;
+    const int R = (int)rank, N = nmodes;
+    const uint64_t I = dims[N - 1], n = I * R;
+    const hipMemcpyKind k = ptr_kind == ALTO_PTR_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    DevBuf g, lam, u, m, ipart, out;
+    ALTO_TRY(g.alloc((size_t)N * R * R * 8));
+    ALTO_TRY(lam.alloc((size_t)R * 8));
+    ALTO_TRY(u.alloc(n * 8));
+    ALTO_TRY(m.alloc(n * 8));
+    ALTO_TRY(ipart.alloc((size_t)64 * R * 8));
+    ALTO_TRY(out.alloc(32));
+    for (int q = 0; q < N; ++q)
+        HIP_TRY(hipMemcpyAsync(g.as<double>() + (size_t)q * R * R, grams[q], (size_t)R * R * 8, k, s));
+    HIP_TRY(hipMemcpyAsync(lam.p, lambda, (size_t)R * 8, k, s));
+    HIP_TRY(hipMemcpyAsync(u.p, U_last, n * 8, k, s));
+    HIP_TRY(hipMemcpyAsync(m.p, U_mttkrp, n * 8, k, s));
+    ALTO_TRY((inner_partial_device<double, double>(u.as<double>(), m.as<double>(), I, R, ipart.as<double>(), s)));
+    ALTO_TRY(fit_device(g.as<double>(), N, R, lam.as<double>(), ipart.as<double>(), normX, out.as<double>(), s));
+    double fb[3];
+    HIP_TRY(hipMemcpyAsync(fb, out.p, 24, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *fit = fb[0];
+    return ALTO_OK;
+}
+
 int alto_pseudo_inverse(const double* grams, int nmodes, int mode, uint64_t rank, double* U, uint64_t dim,
                         int* used_fallback, alto_stream_t stream)
 {

@@ -1545,6 +1545,11 @@ int alto_memcpy_d2h(void* dst, const void* src, uint64_t bytes)
     HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return ALTO_OK;
 }
+int alto_memcpy_d2d(void* dst, const void* src, uint64_t bytes, alto_stream_t stream)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return ALTO_OK;
+}
 int alto_device_trim(void)
 {
     int dev = 0;

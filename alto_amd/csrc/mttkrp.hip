@@ -288,6 +288,151 @@ __global__ void __launch_bounds__(256)
     if (cur != 0xffffffffu) flush_row<double, double, G, E>(a.out, cur, R, gl, acc);
 }
 
+// Phi with 16-byte lanes (the k_mttkrp16 lane mapping): lane gl of a group
+// of G holds columns 2 gl and 2 gl + 1, so every factor row (the model row
+// B[row] and the NO other modes' rows) arrives with one 16-byte load per
+// lane, half the gather instructions of the strided 8-byte lanes.  Needs an
+// even rank and 16-byte aligned factors (rows of 16 R / 2 bytes).  Per
+// nonzero: Pi = the product of the other rows, d = <B[row], Pi> summed over
+// the group, and Pi x / max(d, eps) added into the row's run.  The model
+// row is loaded only where the row changes inside a block of J nonzeros or
+// from the previous block, with the decision uniform over the wave (the
+// stream is ordered by row inside each chunk, so long runs skip most
+// model-row loads).
+template <int G, int PW, int NO>
+__global__ void __launch_bounds__(256)
+    k_phi16(StreamArgs<double, double, double> a, const double* __restrict__ Bm, double eps)
+{
+    constexpr int E = 2;
+    constexpr int NS = G < 4 ? 4 / G : 1;
+    constexpr int BB = G * NS;
+    constexpr int J = 4;
+    const uint64_t block = xcd_block(a.nblocks_real);
+    if (block == ~0ull) return;
+    const int lane = threadIdx.x & 63;
+    const int gl = lane & (G - 1);
+    const uint64_t group = (block * 256 + threadIdx.x) / G;
+    const uint64_t s = group * a.seg;
+    if (s >= a.nnz) return;
+    const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
+    const int R = a.rank;
+    const int c0 = gl * E < R ? gl * E : 0;
+    const bool inr = gl * E < R;
+
+    double acc[E], blast[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) acc[q] = blast[q] = 0;
+    uint32_t cur = 0xffffffffu;
+    uint32_t prev_r = 0xffffffffu;  // row whose model row blast holds
+
+    uint64_t kn[NS][PW];
+    double vn[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const uint64_t i = s + k * G + gl;
+        vn[k] = 0;
+#pragma unroll
+        for (int w = 0; w < PW; ++w) kn[k][w] = 0;
+        if (i < e) {
+#pragma unroll
+            for (int w = 0; w < PW; ++w) kn[k][w] = a.pk[i * PW + w];
+            vn[k] = a.val[i];
+        }
+    }
+    for (uint64_t p = s; p < e; p += BB) {
+        uint32_t offs[NS][NO];
+        uint32_t rown[NS];
+        double myv[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            myv[k] = vn[k];
+#pragma unroll
+            for (int o = 0; o < NO; ++o) offs[k][o] = (uint32_t)(field<PW>(kn[k], a.offo[o], a.bitso[o]) * (uint64_t)R);
+            rown[k] = (uint32_t)field<PW>(kn[k], a.offn, a.bitsn);
+        }
+        const int cnt = (int)(e - p < (uint64_t)BB ? e - p : (uint64_t)BB);
+        static_for<0, BB / J>([&](auto bi) {
+            constexpr int j0 = decltype(bi)::value * J;
+            if (j0 >= cnt) return;
+            double pi[J][E], v[J];
+            uint32_t rj[J];
+            static_for<0, J>([&](auto ji) {
+                constexpr int jj = decltype(ji)::value;
+                constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
+                rj[jj] = bcast32<G, src>(rown[slot]);
+                v[jj] = bcast_val<G, src>(myv[slot]);
+#pragma unroll
+                for (int q = 0; q < E; ++q) pi[jj][q] = 1.0;
+            });
+            static_for<0, NO>([&](auto oi) {
+                constexpr int o = decltype(oi)::value;
+                const double* __restrict__ U = a.Uo[o];
+                static_for<0, J>([&](auto ji) {
+                    constexpr int jj = decltype(ji)::value;
+                    constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
+                    const uint32_t off = bcast32<G, src>(offs[slot][o]);
+                    double x[E];
+                    VecLoad<double, E>::load(U + off + c0, x);
+#pragma unroll
+                    for (int q = 0; q < E; ++q) pi[jj][q] *= x[q];
+                });
+            });
+            // model rows where the row changes (uniform over the wave)
+            double br[J][E];
+            uint32_t ldm = 0;
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const bool chg = jj == 0 ? rj[0] != prev_r : rj[jj] != rj[jj - 1];
+                if (__any(chg)) {
+                    VecLoad<double, E>::load(Bm + (uint64_t)rj[jj] * R + c0, br[jj]);
+                    ldm |= 1u << jj;
+                }
+            }
+            // the mask reaches the selection below through an opaque copy, so
+            // that the selection is not folded into the branches above
+            asm volatile("" : "+s"(ldm));
+            if constexpr (j0 == 0) {
+#pragma unroll
+                for (int k = 0; k < NS; ++k) {
+                    const uint64_t i = p + BB + k * G + gl;
+                    vn[k] = 0;
+#pragma unroll
+                    for (int w = 0; w < PW; ++w) kn[k][w] = 0;
+                    if (i < e) {
+#pragma unroll
+                        for (int w = 0; w < PW; ++w) kn[k][w] = a.pk[i * PW + w];
+                        vn[k] = a.val[i];
+                    }
+                }
+            }
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const bool use = (ldm >> jj) & 1u;
+                double d = 0;
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    blast[q] = use ? br[jj][q] : blast[q];
+                    d += (inr ? blast[q] : 0.0) * pi[jj][q];
+                }
+                d = group_sum<G>(d);
+                const double w = v[jj] / fmax(d, eps);
+                if (j0 + jj < cnt) {
+                    if (rj[jj] != cur) {
+                        if (cur != 0xffffffffu) flush_row<double, double, G, E, true>(a.out, cur, R, gl, acc);
+                        cur = rj[jj];
+#pragma unroll
+                        for (int q = 0; q < E; ++q) acc[q] = 0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < E; ++q) acc[q] += w * pi[jj][q];
+                }
+            }
+            prev_r = rj[J - 1];
+        });
+    }
+    if (cur != 0xffffffffu) flush_row<double, double, G, E, true>(a.out, cur, R, gl, acc);
+}
+
 // ------------------------------------------------------------------ host side
 // factor row offsets (row * rank) are carried as 32-bit values
 static int check_offsets(const alto_tensor* t, int R)
@@ -406,7 +551,19 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
     a.val = t->ms[mode].val64.as<double>();
     a.out = phi;
     dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
-    if (g_phi_variant == 1) {
+    bool al16 = R % 2 == 0 && ((uintptr_t)U[mode] & 15) == 0;
+    for (int o = 0; o < a.nother; ++o) al16 = al16 && ((uintptr_t)a.Uo[o] & 15) == 0;
+    if (g_phi_variant == 2 && al16) {
+        const int no = t->nmode - 1;
+#define PHI16_NO_(N_) hipLaunchKernelGGL((k_phi16<G, PW, N_>), grid, dim3(256), 0, s, a, U[mode], eps)
+        RANK_DISPATCH(R, t->pack.pw, {
+            if (no == 1) PHI16_NO_(1);
+            else if (no == 2) PHI16_NO_(2);
+            else if (no == 3) PHI16_NO_(3);
+            else PHI16_NO_(4);
+        });
+#undef PHI16_NO_
+    } else if (g_phi_variant >= 1) {
         const int no = t->nmode - 1;
 #define PHI_NO_(N_) hipLaunchKernelGGL((k_phi_b<G, E, PW, N_>), grid, dim3(256), 0, s, a, U[mode], eps)
         RANK_DISPATCH(R, t->pack.pw, {
@@ -467,7 +624,7 @@ int alto_tune_set(int param, int value)
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_PHI_VARIANT) {
-        if (value < 0 || value > 1) return ALTO_ERR_ARG;
+        if (value < 0 || value > 2) return ALTO_ERR_ARG;
         g_phi_variant = value;
         return ALTO_OK;
     }

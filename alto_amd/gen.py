@@ -2,8 +2,8 @@
 
 Zipf(s) indices over a seeded random permutation of each mode's range
 (heavy-hitter rows at random positions), duplicates rejected until the target
-number of unique coordinates is reached, values lognormal(0,1) or
-uniform (0,1].  Used by bench.py for the large workloads; the tests use the
+number of unique coordinates is reached (sparse_coo) or kept (sparse_draws),
+values lognormal(0,1) or uniform (0,1].  Used by bench.py for the large workloads; the tests use the
 numpy generators in tests/_tensors.py.
 """
 from __future__ import annotations
@@ -24,6 +24,26 @@ def _draw_mode(cdf, perm, m, gen):
     u = torch.rand(m, dtype=torch.float64, device=cdf.device, generator=gen)
     r = torch.searchsorted(cdf, u).clamp_(max=cdf.numel() - 1)
     return perm[r]
+
+
+def sparse_draws(dims, nnz, seed, s=1.1, values="lognormal", device="cuda"):
+    """nnz Zipf(s) draws per mode over a seeded permutation of each mode's
+    range, duplicate coordinates kept (in COO they add up, as a nonzero
+    listed twice does in the reference): (coords, vals).  One pass of
+    sampling, no deduplication -- the generator of the serving stream's
+    jobs, which are made at submission time."""
+    dims = [int(d) for d in dims]
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    coords = []
+    for d in dims:
+        perm = torch.randperm(d, generator=gen, device=device)
+        coords.append(_draw_mode(_zipf_icdf_table(d, s, device), perm, nnz, gen).contiguous())
+    if values == "lognormal":
+        vals = torch.exp(torch.randn(nnz, dtype=torch.float64, device=device, generator=gen))
+    else:
+        vals = 1.0 - torch.rand(nnz, dtype=torch.float64, device=device, generator=gen)
+    return coords, vals
 
 
 def sparse_coo(dims, nnz, seed, dist="zipf", s=1.1, values="lognormal", device="cuda"):

@@ -133,29 +133,75 @@ def job_stream(n_jobs, seed=5, rate=50.0, nnz_range=(1e4, 2e7), dim_range=(64, 2
     return jobs
 
 
-def materialize(jobs, device="cuda"):
-    """Generates each job's tensor on the GPU (Zipf(1.1) indices, lognormal values)."""
+def _generate(j, device="cuda"):
+    """One job's tensor on the GPU: nnz Zipf(1.1) draws per mode over a
+    seeded permutation of the mode's range (duplicate coordinates kept; in
+    COO they add up) and lognormal values: (coords, vals, nnz)."""
     from . import gen
+    coords, vals = gen.sparse_draws(j["dims"], j["nnz"], j["seed"], s=1.1, values="lognormal", device=device)
+    return coords, vals, int(vals.numel())
+
+
+def materialize(jobs, device="cuda"):
+    """Generates every job's tensor on the GPU at once (see _generate)."""
     for j in jobs:
-        coords, vals = gen.sparse_coo(j["dims"], j["nnz"], j["seed"], dist="zipf", s=1.1, values="lognormal",
-                                      device=device)
-        j["coords"], j["vals"] = coords, vals
-        j["nnz"] = int(vals.numel())
+        j["coords"], j["vals"], j["nnz"] = _generate(j, device)
     return jobs
 
 
-def run_stream(jobs, precision, slots, budget=200 << 30, max_iters=20, epsilon=1e-5, time_scale=1.0):
-    """Submits the jobs at their (scaled) arrival times; returns throughput and latency."""
+def run_stream(jobs, precision, slots, budget=200 << 30, max_iters=20, epsilon=1e-5, time_scale=1.0, window=48):
+    """Submits the jobs at their (scaled) arrival times and returns throughput
+    and latency.  A job without a materialised tensor ("coords") has it
+    generated on the GPU just before its submission and freed as soon as the
+    job finished, and at most `window` generated jobs hold their inputs at
+    once (submission waits for the oldest to finish), so a stream of any
+    length runs in bounded memory.  Latency is submission to result; the
+    generation, before submission, is not part of it."""
+    import collections
+
+    import torch
+
     srv = Server(budget, slots)
-    ids = []
+    ids, nnzs = [], []
+    live = collections.deque()  # (job id, generated inputs) in submission order
+    done = {}
+
+    def retire(block):
+        # frees the inputs of finished jobs, oldest first; with block, waits
+        # for the oldest one
+        while live:
+            jid, _ = live[0]
+            r, _, _ = srv.result(jid, wait=block)
+            if r.status == 0 or r.status == 1:
+                return
+            done[jid] = r
+            live.popleft()
+            block = False
+
     t0 = time.perf_counter()
     for j in jobs:
+        if "coords" in j:
+            coords, vals, nnz = j["coords"], j["vals"], j["nnz"]
+            gen_keep = None
+        else:
+            while len(live) >= window:
+                retire(True)
+            coords, vals, nnz = _generate(j)
+            torch.cuda.current_stream().synchronize()  # the server reads them on its own streams
+            gen_keep = (coords, vals)
         wait = j["arrival"] * time_scale - (time.perf_counter() - t0)
         if wait > 0:
             time.sleep(wait)
-        ids.append(srv.submit(j["dims"], [c.data_ptr() for c in j["coords"]], j["vals"].data_ptr(), j["rank"],
-                              max_iters, epsilon, j["seed"] % (2 ** 31), precision, device=True, nnz=j["nnz"]))
-    res = [srv.result(i)[0] for i in ids]
+        jid = srv.submit(j["dims"], [c.data_ptr() for c in coords], vals.data_ptr(), j["rank"], max_iters, epsilon,
+                         j["seed"] % (2 ** 31), precision, device=True, nnz=nnz)
+        ids.append(jid)
+        nnzs.append(nnz)
+        if gen_keep is not None:
+            live.append((jid, gen_keep))
+            retire(False)
+    while live:
+        retire(True)
+    res = [done[i] if i in done else srv.result(i)[0] for i in ids]
     peak = srv.stats().hbm_peak
     srv.close()
     bad = [r.status for r in res if r.status != 2]
@@ -166,7 +212,8 @@ def run_stream(jobs, precision, slots, budget=200 << 30, max_iters=20, epsilon=1
     last = max(r.t_end for r in res)
     return {"jobs": len(res), "jobs_per_s": len(res) / (last - first), "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "mean_iters": float(np.mean([r.iters for r in res])),
-            "fits": [r.fit for r in res], "hbm_peak_gb": peak / 2 ** 30, "makespan_s": last - first}
+            "fits": [r.fit for r in res], "hbm_peak_gb": peak / 2 ** 30, "makespan_s": last - first,
+            "total_nnz": int(sum(nnzs))}
 
 
 def run_sequential(jobs, precision, max_iters=20, epsilon=1e-5):

@@ -1,15 +1,21 @@
 #!/usr/bin/env python3
-"""CP-ALS throughput of the MI355X ALTO engine on one GPU.
+"""CP-ALS throughput of the MI355X ALTO engine on one GPU, with every
+benchmark workload measured in the same run.
 
-Workload: the 3-mode 12092 x 9184 x 28818 tensor with 76.9M unique
+Headline workload: the 3-mode 12092 x 9184 x 28818 tensor with 76.9M unique
 nonzeros, Zipf(1.1) indices over a seeded permutation of each mode and
 lognormal(0,1) values (synthetic, generated on the GPU), FP64 values and
 factors, 64-bit ALTO keys, rank 32.  A step is one CP-ALS iteration (the
 MTTKRP, pseudo-inverse, normalisation and Gram of every mode plus the fit)
 with the tensor and model resident in HBM.  Printed: one JSON line with
 iterations/s, the all-mode MTTKRP roofline (algorithmic bytes / measured
-kernel time vs 8 TB/s), FP32 / BF16 variants, and the CPU oracle's
-iterations/s on the same tensor (bounded sample: one iteration).
+kernel time vs 8 TB/s, and the HBM traffic of the same launches read from
+the FETCH_SIZE / WRITE_SIZE counters in two rocprofv3 runs of this script),
+FP32 / BF16 variants, the first iteration's fit against the oracle's, and
+the host-core baseline on the same tensor.  Beside it, each other workload:
+the 1000^3 1M-nonzero rank-16 CPU configuration on the GPU and on the host
+cores, CP-APR on 120M Poisson counts (128-bit keys), the 5-mode 200M-nonzero
+rank-64 precision study, the 2000-job serving stream and the batched MTTKRP.
 """
 from __future__ import annotations
 
@@ -33,25 +39,115 @@ NNZ = 76_900_000
 RANK = 32
 SEED = 2
 HBM_PEAK_GBS = 8000.0
+COPY_PROBE_BYTES = 256 << 20
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measured_traffic(nnz):
-    """HBM bytes per MTTKRP launch (read + write) from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this kernel and workload; None when it
-    does not describe this workload."""
-    path = os.path.join(ROOT, "profiles", "k_mttkrp16_nell2shape_zipf_r32_fp64.traffic.json")
+def host_cpu():
+    """(threads this process may use, physical cores of the host, CPU model).
+    The thread count is the process's CPU share: its affinity set, capped by
+    OMP_NUM_THREADS when that is set."""
+    share = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.split(",")[0].isdigit() and int(env.split(",")[0]) > 0:
+        share = min(share, int(env.split(",")[0]))
+    model, cores = "unknown", set()
     try:
-        with open(path) as f:
-            d = json.load(f)
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name":
+                model = v
+            elif k == "physical id":
+                phys = v
+            elif k == "core id":
+                core = v
+            elif not k and phys is not None:
+                cores.add((phys, core))
+                phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
     except OSError:
-        return None
-    if d.get("dims") != DIMS or d.get("nnz") != nnz or d.get("rank") != RANK:
-        return None
-    return d["hbm_read_bytes_per_launch"] + d["hbm_write_bytes_per_launch"]
+        pass
+    return share, len(cores) or os.cpu_count(), model
+
+
+def traffic_probe():
+    """Profiling driver for the counter runs: a COPY_PROBE_BYTES device copy
+    (the calibration kernel), then the headline tensor and three all-mode
+    FP64 MTTKRP passes (the first also builds the mode streams)."""
+    import torch
+
+    import alto_amd as A
+    from alto_amd import gen
+
+    a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    coords, vals = gen.sparse_coo(DIMS, NNZ, SEED, dist="zipf", s=1.1, values="lognormal")
+    T = A.AltoTensor.from_device(DIMS, [c.data_ptr() for c in coords], vals.data_ptr(), NNZ, nprtn=16)
+    del coords, vals
+    U0, _ = A.kruskal_random_init(DIMS, RANK, SEED)
+    Uf = [torch.from_numpy(u).cuda() for u in U0]
+    outs = [torch.zeros(d, RANK, dtype=torch.float64, device="cuda") for d in DIMS]
+    uptr = A._ptr_array([u.data_ptr() for u in Uf])
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(3):
+        for m in range(3):
+            A.check(A.lib().alto_mttkrp(T.h, m, RANK, uptr, outs[m].data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+    torch.cuda.synchronize()
+
+
+def measure_traffic():
+    """HBM bytes per all-mode-pass MTTKRP call (mean over the three modes),
+    read (FETCH_SIZE) and written (WRITE_SIZE), from two rocprofv3 counter
+    runs of traffic_probe() in child processes.  Each counter is scaled by
+    what it reads for the device copy of known size (FETCH_SIZE counts a
+    copy's reads at half: scale 2.0 on this GPU).  The MTTKRP dispatches of
+    the last pass (its kernels' names contain "mttkrp") are summed.  Returns
+    a dict, or a dict with "error" when the profiler is missing or fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return {"error": "rocprofv3 not found"}
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="alto_traffic_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "240", prof, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv",
+               "--", sys.executable, os.path.join(ROOT, "bench.py"), "--traffic-probe"]
+        t0 = time.time()
+        r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), stdout=subprocess.DEVNULL,
+                           stderr=subprocess.PIPE, text=True)
+        files = [os.path.join(root, f) for root, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        log(f"traffic: rocprofv3 --pmc {counter}: exit {r.returncode} in {time.time() - t0:.1f}s")
+        if not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return {"error": f"rocprofv3 --pmc {counter} wrote no counters (exit {r.returncode}): "
+                             f"{r.stderr.strip().splitlines()[-1:] if r.stderr else ''}"}
+        rows = sorted(csv.DictReader(open(files[0])), key=lambda x: int(x["Dispatch_Id"]))
+        shutil.rmtree(d, ignore_errors=True)
+        copies = [float(x["Counter_Value"]) for x in rows if "copyBuffer" in x["Kernel_Name"]]
+        mtt = [float(x["Counter_Value"]) for x in rows if "mttkrp" in x["Kernel_Name"]]
+        if not copies or not mtt or len(mtt) % 3:
+            return {"error": f"{counter}: {len(copies)} copy and {len(mtt)} MTTKRP dispatches"}
+        scale = (COPY_PROBE_BYTES / 1024.0) / copies[-1]
+        last = mtt[-len(mtt) // 3:]  # the last all-mode pass
+        out[counter] = {"kib_last_pass": sum(last), "copy_kib": copies[-1], "scale": scale,
+                        "bytes_per_call": sum(last) * 1024.0 * scale / 3, "dispatches_per_pass": len(last)}
+    rd, wr = out["FETCH_SIZE"]["bytes_per_call"], out["WRITE_SIZE"]["bytes_per_call"]
+    return {"read_bytes_per_call": rd, "write_bytes_per_call": wr, "bytes_per_call": rd + wr,
+            "fetch_scale": out["FETCH_SIZE"]["scale"], "write_scale": out["WRITE_SIZE"]["scale"],
+            "dispatches_per_pass": out["FETCH_SIZE"]["dispatches_per_pass"]}
 
 
 def main():
@@ -60,20 +156,26 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nnz", type=int, default=NNZ)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host-core baselines")
     ap.add_argument("--no-variants", action="store_true", help="skip the FP32/BF16 runs")
-    ap.add_argument("--apr", action="store_true",
-                    help="also run CP-APR on the 2^20 x 2^18 x 2^16 x 1024 Poisson count tensor (rank 10, 30 x 10)")
-    ap.add_argument("--precision-study", action="store_true",
-                    help="also run the 5-mode 50K x 50K x 10K x 2K x 512 Zipf(1.2) FP64 / FP32 / BF16 study (rank 64)")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 1000^3 1M-nonzero rank-16 workload")
+    ap.add_argument("--no-apr", action="store_true",
+                    help="skip CP-APR on the 2^20 x 2^18 x 2^16 x 1024 Poisson count tensor (rank 10, 30 x 10)")
+    ap.add_argument("--no-precision-study", action="store_true",
+                    help="skip the 5-mode 50K x 50K x 10K x 2K x 512 Zipf(1.2) FP64 / FP32 / BF16 study (rank 64)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter runs")
+    ap.add_argument("--apr", action="store_true", help=argparse.SUPPRESS)  # on by default
+    ap.add_argument("--precision-study", action="store_true", help=argparse.SUPPRESS)  # on by default
     ap.add_argument("--no-autotune", action="store_true", help="use the default MTTKRP launch shapes")
     ap.add_argument("--mttkrp-only", type=int, default=0, metavar="REPS",
                     help="profiling driver: build the tensor, run REPS all-mode MTTKRPs, print nothing else")
     ap.add_argument("--copy-only", action="store_true",
                     help="profiling calibration: a 256 MiB device copy (for FETCH_SIZE / WRITE_SIZE scaling)")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="profiling driver of the traffic measurement: the calibration copy, then 3 all-mode MTTKRPs")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling driver: only the 16-job batched MTTKRP comparison, its JSON on stdout")
-    ap.add_argument("--serve-jobs", type=int, default=100,
+    ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the Poisson(50/s) serving stream to run (0 = skip; the full stream is 2000)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -96,6 +198,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         return
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.batched_only:
         print(json.dumps(batched_leg()), flush=True)
         return
@@ -112,15 +217,35 @@ def main():
     log(f"ALTO creation time:   {create_s:f}")
 
     U0, _ = A.kruskal_random_init(DIMS, RANK, SEED)
+    Uf = [torch.from_numpy(u).cuda() for u in U0]
+    outs = [torch.zeros(d, RANK, dtype=torch.float64, device="cuda") for d in DIMS]
+    uptr = A._ptr_array([u.data_ptr() for u in Uf])
+
+    def allmode():
+        for m in range(3):
+            A.check(A.lib().alto_mttkrp(T.h, m, RANK, uptr, outs[m].data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+
     if args.mttkrp_only:
-        Uf = [torch.from_numpy(u).cuda() for u in U0]
-        outs = [torch.zeros(d, RANK, dtype=torch.float64, device="cuda") for d in DIMS]
-        uptr = A._ptr_array([u.data_ptr() for u in Uf])
         for _ in range(args.mttkrp_only):
-            for m in range(3):
-                A.check(A.lib().alto_mttkrp(T.h, m, RANK, uptr, outs[m].data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+            allmode()
         torch.cuda.synchronize()
         return
+    # the first MTTKRP of each mode also builds that mode's stream (sorted
+    # copy of the nonzeros); set-up = format + streams, the steady all-mode
+    # time taken out
+    bytes_format = T.info().device_bytes
+    t0 = time.time()
+    allmode()
+    torch.cuda.synchronize()
+    first_pass_s = time.time() - t0
+    t0 = time.time()
+    allmode()
+    torch.cuda.synchronize()
+    steady_pass_s = time.time() - t0
+    streams_s = max(first_pass_s - steady_pass_s, 0.0)
+    bytes_streams = T.info().device_bytes
+    log(f"mode streams built in {streams_s:.3f}s; device bytes {bytes_format / 1e9:.2f} GB (format) -> "
+        f"{bytes_streams / 1e9:.2f} GB (with streams)")
 
     def run_als(prec, steps, warmup):
         sess = A.CpdSession(T, U0, precision=prec, stream=stream)
@@ -143,14 +268,15 @@ def main():
         log(f"autotune fp64 ({time.time() - t0:.1f}s): " +
             ", ".join(f"mode {e['mode']}: v{e['variant']} seg {e['seg']} {e['ms']:.3f} ms (default {e['default_ms']:.3f})"
                       for e in tuned["fp64"]))
+    # iteration 1 from the initial model, for the check against the oracle
+    s1 = A.CpdSession(T, U0, precision=A.PREC_F64, stream=stream)
+    fit1_gpu = float(s1.iterate(1)[0][0])
+    del s1
     dt, fits, tm = run_als(A.PREC_F64, args.steps, args.warmup)
     ms_per_step = dt / args.steps * 1e3
     log(f"FP64 CP-ALS: {args.steps} iterations in {dt:.3f}s ({ms_per_step:.3f} ms/iter), fit {fits[-1]:.6f}")
 
-    # all-mode MTTKRP roofline: algorithmic bytes per launch / kernel time
-    Uf = [torch.from_numpy(u).cuda() for u in U0]
-    outs = [torch.zeros(d, RANK, dtype=torch.float64, device="cuda") for d in DIMS]
-    uptr = A._ptr_array([u.data_ptr() for u in Uf])
+    # all-mode MTTKRP roofline: algorithmic bytes per launch / kernel time;
     # each launch bracketed by its own events on the launch stream (torch's
     # current stream): the average gives `achieved`, the median is reported
     reps = 20
@@ -182,20 +308,33 @@ def main():
                 tuned[name] = T.mttkrp_autotune(RANK, prec, reps=3, stream=stream)
             vdt, vfits, _ = run_als(prec, args.steps, args.warmup)
             variants[name] = {"iters_per_s": args.steps / vdt, "ms_per_step": vdt / args.steps * 1e3,
-                              "fit": float(vfits[-1]), "fit_delta_vs_fp64": float(abs(vfits[-1] - fits[-1]))}
+                              "fit": float(vfits[-1]), "fit_delta_vs_fp64_gpu": float(abs(vfits[-1] - fits[-1]))}
             log(f"{name} CP-ALS: {vdt / args.steps * 1e3:.3f} ms/iter, fit {vfits[-1]:.6f}")
 
-    apr = apr_leg() if args.apr else None
-    prec_study = precision_leg() if args.precision_study else None
-
-    serving = None
-    if args.serve_jobs > 0:
-        serving = serve_leg(args.serve_jobs)
-    batched = batched_leg() if args.serve_jobs > 0 else None
-
-    cpu = None
+    cpu = fit1 = None
     if not args.no_cpu:
-        cpu = cpu_baseline(T, U0)
+        cpu, fit1_oracle = cpu_baseline(T, U0)
+        fit1 = {"gpu": fit1_gpu, "oracle": fit1_oracle, "abs_diff": abs(fit1_gpu - fit1_oracle)}
+        log(f"iteration-1 fit: GPU {fit1_gpu:.15f} oracle {fit1_oracle:.15f} (|diff| {fit1['abs_diff']:.1e})")
+    del T
+    torch.cuda.empty_cache()
+
+    c1 = c1_leg(not args.no_cpu) if not args.no_c1 else None
+    apr = apr_leg() if not args.no_apr else None
+    prec_study = precision_leg() if not args.no_precision_study else None
+    serving = batched = None
+    if args.serve_jobs > 0:
+        serving = serve_leg(args.serve_jobs, not args.no_cpu)
+        batched = batched_leg()
+    traffic = None
+    if not args.no_traffic:
+        traffic = measure_traffic()
+        if "error" in traffic:
+            log(f"traffic: {traffic['error']}")
+        else:
+            log(f"traffic: {traffic['read_bytes_per_call'] / 1e9:.3f} GB read + "
+                f"{traffic['write_bytes_per_call'] / 1e9:.3f} GB written per MTTKRP call "
+                f"(algorithmic {np.mean(mode_bytes) / 1e9:.3f} GB)")
 
     line = {
         "metric": "CP-ALS iters/s + MTTKRP HBM GB/s (frac peak); batched jobs/s, p99 latency",
@@ -213,22 +352,112 @@ def main():
         "config": {"workload": "cp-als nell2-shape zipf", "dims": DIMS, "nnz": nnz, "rank": RANK,
                    "alto_key_bits": 64, "modes": 3, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(nnz),
-                     "kernel": "k_mttkrp16 (all-mode, 3 launches)", "ms_per_launch": mode_ms,
-                     "median_ms_per_launch": mode_med_ms, "launches_timed": reps,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_call"] if traffic and "error" not in traffic else None,
+                     "traffic_detail": traffic,
+                     "kernel": "alto_mttkrp FP64 (k_mttkrp_fib long-fiber part + k_mttkrp16 rest), all modes",
+                     "ms_per_launch": mode_ms, "median_ms_per_launch": mode_med_ms, "launches_timed": reps,
                      "algorithmic_bytes_per_launch": mode_bytes, "measured_copy_GBps": copy_gbs},
         "mttkrp_allmode_ms": allmode_ms,
         "autotune": tuned,
         "alto_create_s": create_s,
+        "alto_create_with_streams_s": create_s + streams_s,
+        "device_bytes": {"format": bytes_format, "with_mode_streams": bytes_streams},
         "cpd_timers_s": tm,
+        "iteration1_fit_vs_oracle": fit1,
         "precision_variants": variants,
-        "serving": serving,
-        "batched_mttkrp": batched,
+        "c1_cpu_config": c1,
         "cp_apr": apr,
         "precision_study": prec_study,
+        "serving": serving,
+        "batched_mttkrp": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def c1_leg(with_cpu):
+    """The 1000 x 1000 x 1000 tensor with 1M unique uniform coordinates and
+    uniform (0,1] values, 64-bit keys, rank 16: 20 CP-ALS iterations and 100
+    mode-0 MTTKRPs (the command line's -p -t 0) on the GPU; on the host
+    cores, the same MTTKRPs through the reference's own mttkrp_alto_par
+    (oracle/_ref, built from the reference sources) and the 20 iterations
+    through the oracle's C restatement."""
+    import torch
+
+    import alto_amd as A
+    from alto_amd import gen
+
+    dims, nnz, R = [1000, 1000, 1000], 1_000_000, 16
+    coords, vals = gen.sparse_coo(dims, nnz, 1, dist="uniform", values="uniform")
+    T = A.AltoTensor.from_device(dims, [c.data_ptr() for c in coords], vals.data_ptr(), nnz, nprtn=16)
+    del coords, vals
+    U0, _ = A.kruskal_random_init(dims, R, 1)
+    stream = torch.cuda.current_stream().cuda_stream
+    sess = A.CpdSession(T, U0, precision=A.PREC_F64, stream=stream)
+    sess.iterate(1)
+    del sess
+    sess = A.CpdSession(T, U0, precision=A.PREC_F64, stream=stream)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    fits, _ = sess.iterate(20, epsilon=-1.0)
+    torch.cuda.synchronize()
+    als_s = time.perf_counter() - t
+    del sess
+    Uf = [torch.from_numpy(u).cuda() for u in U0]
+    out = torch.zeros(dims[0], R, dtype=torch.float64, device="cuda")
+    uptr = A._ptr_array([u.data_ptr() for u in Uf])
+    A.check(A.lib().alto_mttkrp(T.h, 0, R, uptr, out.data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(100):
+        A.check(A.lib().alto_mttkrp(T.h, 0, R, uptr, out.data_ptr(), A.PREC_F64, 1, stream), "mttkrp")
+    e1.record()
+    torch.cuda.synchronize()
+    mtt_s = e0.elapsed_time(e1) * 1e-3
+    res = {"dims": dims, "nnz": nnz, "rank": R,
+           "gpu": {"cpals_20_iters_s": als_s, "iters_per_s": 20 / als_s, "final_fit": float(fits[-1]),
+                   "mttkrp_mode0_100_s": mtt_s, "mttkrp_mode0_ms": mtt_s * 10}}
+    log(f"c1: GPU 20 CP-ALS iterations {als_s * 1e3:.1f} ms, 100 mode-0 MTTKRPs {mtt_s * 1e3:.1f} ms")
+    if with_cpu:
+        import oracle
+        threads, cores, model = host_cpu()
+        oracle.lib().or_set_threads(threads)
+        st = T.export()
+        o = oracle.OracleAlto.wrap(dims, st["idx"].shape[1], T.nprtn, st["masks"], st["idx"], st["vals"],
+                                   st["prtn_ptr"], st["intervals"], st["cr_masks"])
+        U = [u.copy() for u in U0]
+        lam = np.ones(R)
+        t = time.time()
+        _, ofits = o.cpd_als(U, lam, 20, -1.0)
+        c_als = time.time() - t
+        cpu = {"threads": threads, "host_physical_cores": cores, "cpu_model": model,
+               "oracle_cpals_20_iters_s": c_als, "oracle_final_fit": float(ofits[-1]),
+               "final_fit_abs_diff_gpu_vs_oracle": abs(float(ofits[-1]) - float(fits[-1]))}
+        W = st["idx"].shape[1]
+        inf = T.info()
+        if oracle.ref_available(W):
+            with _HostStdoutToNull():
+                r = oracle.RefAlto.wrap(dims, W, T.nprtn, st["masks"], np.array(inf.alto_mask[:W], np.uint64),
+                                        st["idx"], st["vals"], st["prtn_ptr"], st["intervals"], st["cr_masks"],
+                                        np.array(inf.alto_cr_mask[:W], np.uint64))
+                r.mttkrp(0, U0)
+                t = time.time()
+                for _ in range(100):
+                    r.mttkrp(0, U0)
+                cpu["reference_mttkrp_mode0_100_s"] = time.time() - t
+                del r
+        else:
+            t = time.time()
+            for _ in range(100):
+                o.mttkrp(0, U0)
+            cpu["oracle_mttkrp_mode0_100_s"] = time.time() - t
+        res["cpu"] = cpu
+        log(f"c1: host {threads} threads: oracle 20 iterations {c_als:.2f}s, 100 mode-0 MTTKRPs "
+            f"{cpu.get('reference_mttkrp_mode0_100_s', cpu.get('oracle_mttkrp_mode0_100_s')):.2f}s")
+    del T
+    torch.cuda.empty_cache()
+    return res
 
 
 def apr_leg():
@@ -319,16 +548,37 @@ def precision_leg():
             per.append({"ms": e0.elapsed_time(e1) / 3, "rel_err_vs_fp64": err})
         res["mttkrp"][name] = per
         del F
+    # the fit of this workload is not a usable measure (rank 64 cannot fit the
+    # 200M-nonzero tensor: the residual formula goes negative and the fit far
+    # above 1), so the precisions are compared on the model itself: per mode,
+    # ||U_p - U_fp64||_F / ||U_fp64||_F after the same k iterations (k = 1, 5,
+    # 25; the iteration is not contracting on this tensor, so differences
+    # grow with k), and the same for lambda
+    checkpoints = (1, 5, 25)
+    models = {}
     for name, prec in (("fp64", A.PREC_F64), ("fp32", A.PREC_F32), ("bf16", A.PREC_BF16)):
         sess = A.CpdSession(T, U0, precision=prec, stream=stream)
-        t = time.perf_counter()
-        fits, _ = sess.iterate(25, epsilon=-1.0)
-        dt = time.perf_counter() - t
-        res[name + "_cpals"] = {"iters_per_s": 25 / dt, "final_fit": float(fits[-1])}
+        dt, done, fits, models[name] = 0.0, 0, [], {}
+        for k in checkpoints:
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            f, _ = sess.iterate(k - done, epsilon=-1.0)
+            torch.cuda.synchronize()
+            dt += time.perf_counter() - t
+            fits += list(f)
+            done = k
+            models[name][k] = sess.get()[:2]
+        res[name + "_cpals"] = {"iters_per_s": done / dt, "final_fit": float(fits[-1])}
         del sess
     for name in ("fp32", "bf16"):
-        res[name + "_cpals"]["final_fit_delta_vs_fp64"] = abs(res[name + "_cpals"]["final_fit"] -
-                                                              res["fp64_cpals"]["final_fit"])
+        r = res[name + "_cpals"]
+        for k in checkpoints:
+            U64, l64 = models["fp64"][k]
+            Up, lp = models[name][k]
+            r[f"factor_rel_diff_vs_fp64_after_{k}"] = [
+                float(np.linalg.norm(Up[m] - U64[m]) / np.linalg.norm(U64[m])) for m in range(len(dims))]
+            r[f"lambda_rel_diff_vs_fp64_after_{k}"] = float(np.linalg.norm(lp - l64) / np.linalg.norm(l64))
+        r["final_fit_delta_vs_fp64"] = abs(r["final_fit"] - res["fp64_cpals"]["final_fit"])
     log(f"precision: {json.dumps(res)}")
     del T
     torch.cuda.empty_cache()
@@ -354,45 +604,79 @@ def device_copy_gbs(nbytes=1 << 30, reps=10):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
-def serve_leg(n_jobs):
+def serve_leg(n_jobs, with_cpu):
     """The job stream (Poisson 50/s, 3-5 modes, dims log-uniform [64, 2^20],
     nnz log-uniform [1e4, 2e7], Zipf(1.1), rank in {8,16,32,64}, 20 iterations
     or fit change < 1e-5) under a 200 GB HBM budget: the plain FP64 per-job path
     (one job at a time) against the batched server (8 concurrent jobs) in FP32
-    and BF16."""
+    and BF16, at the stream's arrival times and saturated (every job
+    available at once).  Each job's tensor is generated on the GPU just
+    before its submission and freed when it finished.  With with_cpu, the
+    first jobs of the stream also run through the oracle on the host cores
+    (a bounded sample)."""
     import torch
 
     from alto_amd import PREC_BF16, PREC_F32, PREC_F64
-    from alto_amd.serve import job_stream, materialize, run_stream
+    from alto_amd.serve import job_stream, run_stream
 
-    t0 = time.time()
-    jobs = materialize(job_stream(n_jobs, seed=5))
-    torch.cuda.synchronize()
-    log(f"serving: generated {n_jobs} jobs ({sum(j['nnz'] for j in jobs) / 1e6:.0f}M nonzeros) in {time.time() - t0:.1f}s")
-    out = {"jobs": n_jobs, "arrival_rate_per_s": 50.0, "hbm_budget_gb": 200}
-    for name, prec, slots in (("fp64_per_job", PREC_F64, 1), ("fp32_batched", PREC_F32, 8),
-                              ("bf16_batched", PREC_BF16, 8)):
-        r = run_stream(jobs, prec, slots)
+    jobs = job_stream(n_jobs, seed=5)
+    out = {"jobs": n_jobs, "arrival_rate_per_s": 50.0, "hbm_budget_gb": 200,
+           "stream_nnz_requested": int(sum(j["nnz"] for j in jobs))}
+    for name, prec, slots, scale in (("fp64_per_job", PREC_F64, 1, 1.0), ("fp32_batched", PREC_F32, 8, 1.0),
+                                     ("bf16_batched", PREC_BF16, 8, 1.0),
+                                     ("fp64_per_job_saturated", PREC_F64, 1, 0.0),
+                                     ("fp32_batched_saturated", PREC_F32, 8, 0.0),
+                                     ("bf16_batched_saturated", PREC_BF16, 8, 0.0)):
+        t0 = time.time()
+        r = run_stream(jobs, prec, slots, time_scale=scale)
         fits = r.pop("fits")
         out[name] = r
-        log(f"serving {name}: {r['jobs_per_s']:.1f} jobs/s p50 {r['p50_latency_s'] * 1e3:.1f} ms "
-            f"p99 {r['p99_latency_s'] * 1e3:.1f} ms")
         out[name + "_mean_fit"] = float(np.mean(fits))
-    for name in ("fp32_batched", "bf16_batched"):
-        out[name + "_speedup_vs_fp64"] = out[name]["jobs_per_s"] / out["fp64_per_job"]["jobs_per_s"]
-    # saturation capacity: every job available at t = 0
-    for name, prec, slots in (("fp64_per_job_saturated", PREC_F64, 1), ("fp32_batched_saturated", PREC_F32, 8),
-                              ("bf16_batched_saturated", PREC_BF16, 8)):
-        r = run_stream(jobs, prec, slots, time_scale=0.0)
-        r.pop("fits")
-        out[name] = r
-        log(f"serving {name}: {r['jobs_per_s']:.1f} jobs/s")
+        log(f"serving {name}: {r['jobs']} jobs ({r['total_nnz'] / 1e9:.2f}G nonzeros) {r['jobs_per_s']:.1f} jobs/s "
+            f"p50 {r['p50_latency_s'] * 1e3:.1f} ms p99 {r['p99_latency_s'] * 1e3:.1f} ms ({time.time() - t0:.0f}s)")
     for name in ("fp32", "bf16"):
+        out[name + "_batched_speedup_vs_fp64"] = (out[name + "_batched"]["jobs_per_s"] /
+                                                  out["fp64_per_job"]["jobs_per_s"])
         out[name + "_saturated_speedup_vs_fp64"] = (out[name + "_batched_saturated"]["jobs_per_s"] /
                                                     out["fp64_per_job_saturated"]["jobs_per_s"])
-    del jobs
+    if with_cpu:
+        out["cpu_sample"] = cpu_serving_sample(jobs)
     torch.cuda.empty_cache()
     return out
+
+
+def cpu_serving_sample(jobs, budget_s=20.0):
+    """The stream's first jobs through the oracle's CP-ALS on the host cores
+    (same tensor, generated on the GPU and copied over; same rank, 20
+    iterations or fit change < 1e-5), until budget_s of host time is spent;
+    jobs/s of that sample."""
+    import oracle
+    from alto_amd.serve import _generate
+
+    threads, cores, model = host_cpu()
+    oracle.lib().or_set_threads(threads)
+    done, spent, nnz = 0, 0.0, 0
+    for j in jobs:
+        coords, vals, n = _generate(j)
+        cidx = np.stack([c.cpu().numpy().astype(np.uint64) for c in coords])
+        v = vals.cpu().numpy()
+        del coords, vals
+        t = time.time()
+        o = oracle.OracleAlto(j["dims"], cidx, v, W=1 if sum(max((d - 1).bit_length(), 1) for d in j["dims"]) <= 64
+                              else 2, nprtn=threads)
+        U, lam = oracle.random_init(j["dims"], j["rank"], j["seed"] % (2 ** 31))
+        o.cpd_als(U, lam, 20, 1e-5)
+        spent += time.time() - t
+        done += 1
+        nnz += n
+        del o
+        if spent >= budget_s:
+            break
+    log(f"serving CPU sample: {done} jobs ({nnz / 1e6:.1f}M nonzeros) in {spent:.1f}s on {threads} threads")
+    return {"jobs": done, "nnz": nnz, "host_s": spent, "jobs_per_s": done / spent if spent else None,
+            "threads": threads, "host_physical_cores": cores, "cpu_model": model, "kind": "port",
+            "sample": f"the stream's first {done} jobs (ALTO build + CP-ALS, 20 iterations or fit change < 1e-5) "
+                      f"through the oracle, no arrival pacing"}
 
 
 def batched_leg(n_jobs=16, reps=50):
@@ -514,8 +798,9 @@ class _HostStdoutToNull:
 
 
 def cpu_baseline(T, U0):
-    """One CP-ALS iteration on the host cores, on the same tensor exported
-    from the GPU.  With the reference harness built (oracle/_ref: the
+    """One CP-ALS iteration on the host cores (this process's CPU share), on
+    the same tensor exported from the GPU; returns (baseline, the
+    iteration's fit).  With the reference harness built (oracle/_ref: the
     reference's include/alto.hpp compiled from its sources), the iteration's
     MTTKRPs are the reference's own mttkrp_alto_par and the rest (R x R
     solves, norms, Grams, fit: the reference's driver needs BLAS / LAPACK,
@@ -525,7 +810,7 @@ def cpu_baseline(T, U0):
 
     import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads, cores, model = host_cpu()
     oracle.lib().or_set_threads(threads)
     st = T.export()
     inf = T.info()
@@ -539,11 +824,12 @@ def cpu_baseline(T, U0):
     dt = time.time() - t1
     log(f"CPU oracle: 1 CP-ALS iteration in {dt:.2f}s on {threads} threads (wrap {t1 - t0:.1f}s)")
     port = {"value": 1.0 / dt, "unit": "CP-ALS iterations/s", "cores": threads, "kind": "port",
+            "host_physical_cores": cores, "cpu_model": model,
             "sample": f"1 CP-ALS iteration (rank {RANK}) on the full {inf.nnz}-nonzero tensor, "
-                      f"OpenMP over {threads} threads, nprtn {T.nprtn}"}
+                      f"OpenMP over {threads} threads (this process's CPU share), nprtn {T.nprtn}"}
     W = st["idx"].shape[1]
     if not oracle.ref_available(W):
-        return port
+        return port, float(fit[0])
     t2 = time.time()
     for m in range(3):
         o.mttkrp(m, U)
@@ -562,11 +848,12 @@ def cpu_baseline(T, U0):
     log(f"CPU reference: all-mode mttkrp_alto_par {t_ref_mttkrp:.3f}s (oracle {t_oracle_mttkrp:.3f}s); "
         f"iteration {t_iter:.3f}s")
     return {"value": 1.0 / t_iter, "unit": "CP-ALS iterations/s", "cores": threads, "kind": "reference",
+            "host_physical_cores": cores, "cpu_model": model,
             "sample": f"1 CP-ALS iteration (rank {RANK}) on the full {inf.nnz}-nonzero tensor, OpenMP over {threads} "
                       f"threads, nprtn {T.nprtn}: the reference's mttkrp_alto_par for the three MTTKRPs "
                       f"({t_ref_mttkrp:.3f} s) plus the oracle's solves, norms, Grams and fit "
                       f"({max(dt - t_oracle_mttkrp, 0.0):.3f} s)",
-            "reference_allmode_mttkrp_s": t_ref_mttkrp, "port": port}
+            "reference_allmode_mttkrp_s": t_ref_mttkrp, "port": port}, float(fit[0])
 
 
 if __name__ == "__main__":

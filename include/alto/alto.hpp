@@ -1,22 +1,26 @@
-// Drop-in C++ API of the reference ALTO library (IntelLabs/ALTO) over
-// libalto_hip.  The names, parameter order, data layouts and in-place
-// semantics follow the reference headers; the work runs on the GPU.
+// ALTO tensors of the reference library (include/alto.hpp) over libalto_hip.
+// The names, parameter order, data layouts and in-place semantics follow the
+// reference; linearization, sorting, partitioning and the MTTKRP run on the
+// GPU, where the tensor's nonzeros live (AltoTensor::dev).
 //
 //   reference                                   here
-//   SparseTensor      (include/sptensor.hpp:8)  same fields
-//   KruskalModel      (include/kruskal_model.hpp:8)  same fields
-//   AltoTensor<LIT>   (include/alto.hpp:27)     same scalar fields + device handle
+//   AltoTensor<LIT>   (include/alto.hpp:27-54)  same fields + device handle
 //   create_alto       (include/alto.hpp:108)    device linearize / sort / partition
+//   destroy_alto      (include/alto.hpp:296)    frees host fields and device tensor
+//   create_da_mem / destroy_da_mem (:324-398)   no per-partition buffers needed
 //   mttkrp_alto[_par] (include/alto.hpp:457,617) device MTTKRP, adds into factors[mode]
 //   mttkrp_alto_batched (none)                  many tensors' MTTKRPs in one launch
-//   cpd_alto          (include/cpd.hpp:43)      device CP-ALS, updates M in place
-//   cp_apr_alto       (include/apr.hpp:984)     device CP-APR, updates M in place
 //
-// Errors: the reference asserts / exits; here a failed call prints
-// alto_last_error() and aborts, so a misuse stops the program as there.
+// The host copies idx / vals / oidx that the reference's structure carries
+// are filled by create_alto from the device tensor (the reference builds
+// them on the host); define ALTO_DROPIN_DEVICE_ONLY to leave them NULL and
+// skip the copies, for programs that only hand the tensor to the device.
+// Including this header also declares the model, COO tensor and common
+// helpers (kruskal_model.hpp, sptensor.hpp, common.hpp).
 #ifndef ALTO_DROPIN_ALTO_HPP_
 #define ALTO_DROPIN_ALTO_HPP_
 
+#include <assert.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -25,68 +29,44 @@
 #include <memory>
 #include <vector>
 
-#include "../alto_hip.h"
+#include "common.hpp"
+#include "device.hpp"
+#include "kruskal_model.hpp"
+#include "sptensor.hpp"
 
-#ifndef MAX_NUM_MODES
-#define MAX_NUM_MODES 5
+// include/alto.hpp:90: modes with at most this many nonzeros per row get an
+// output-ordered index (oidx)
+#ifndef MIN_FIBER_REUSE
+#define MIN_FIBER_REUSE 4
 #endif
 
-typedef unsigned long long IType;  // include/common.hpp:38-42
-typedef double FType;              // include/common.hpp:45-58
-#ifdef VAL_TYPE_INT64
-typedef unsigned long long ValType;
-#else
-typedef FType ValType;
-#endif
-
-typedef enum Model_ { ALS, APR } Model;
-
-struct SparseTensor {  // include/sptensor.hpp:8-26
-    int nmodes;
-    IType* dims;
-    IType nnz;
-    IType** cidx;
-    FType* vals;
-};
-
-struct KruskalModel {  // include/kruskal_model.hpp:8-15
-    int mode;
-    IType* dims;
-    IType rank;
-    FType** U;
-    FType* lambda;
-};
-
+// include/alto.hpp:22-25
 typedef struct Interval_ {
     IType start;
     IType stop;
 } Interval;
 
 template <typename LIT>
-struct AltoTensor {  // include/alto.hpp:27-54 (device-resident nonzeros)
-    int nmode;
-    int nprtn;
-    IType nnz;
-    IType* dims;
+struct AltoTensor {  // include/alto.hpp:27-54
+    int nmode;                  // number of modes
+    int nprtn;                  // number of partitions
+    IType nnz;                  // number of nonzeros
+    IType* dims;                // dimensions
     LIT alto_mask;
-    LIT* mode_masks;
-    IType* prtn_ptr;
-    Interval* prtn_intervals;
+    LIT* mode_masks;            // gather/scatter masks (nmode)
+    LIT* idx;                   // ALTO keys, CR bits included (host copy)
+    ValType* vals;              // nonzeros in key order (host copy)
+    IType* prtn_ptr;            // nprtn + 1 partition bounds
+    Interval* prtn_intervals;   // per-partition index intervals (nprtn * nmode)
     LIT alto_cr_mask;
-    LIT* cr_masks;
-    bool is_oidx;
-    alto_tensor_t* dev;  // the device tensor (keys, values, per-mode streams)
+    LIT* cr_masks;              // conflict resolution masks (nmode)
+    FType* precomp_row;         // unused: CP-APR keeps its buffers on the device
+    bool is_oidx;               // is oidx data available?
+    IType** oidx;               // positions by row index, for modes that need it
+    alto_tensor_t* dev;         // the device tensor (keys, values, per-mode streams)
 };
 
 namespace alto_dropin {
-
-inline void check(int status, const char* what)
-{
-    if (status != ALTO_OK) {
-        fprintf(stderr, "%s failed: %s\n", what, alto_last_error());
-        abort();
-    }
-}
 
 template <typename LIT> inline int lit_bits() { return (int)(8 * sizeof(LIT)); }
 
@@ -98,48 +78,46 @@ template <typename LIT> inline LIT lit_from(const uint64_t* w, int words)
     return v;
 }
 
-// device copies of host factor matrices for one call
-struct DeviceFactors {
-    std::vector<void*> p;
-    DeviceFactors(int nmodes, const IType* dims, IType rank, FType** U)
-    {
-        p.resize(nmodes);
-        for (int m = 0; m < nmodes; ++m) {
-            check(alto_device_malloc(&p[m], dims[m] * rank * sizeof(FType)), "alto_device_malloc");
-            check(alto_memcpy_h2d(p[m], U[m], dims[m] * rank * sizeof(FType)), "alto_memcpy_h2d");
-        }
+template <typename T> inline T* amalloc(size_t n)
+{
+    T* p = (T*)AlignedMalloc(n * sizeof(T));
+    if (!p) {
+        fprintf(stderr, "out of host memory (%zu bytes)\n", n * sizeof(T));
+        abort();
     }
-    ~DeviceFactors()
-    {
-        for (void* q : p) alto_device_free(q);
-    }
-};
+    return p;
+}
 
 }  // namespace alto_dropin
 
 // ---------------------------------------------------------------- ALTO format
+// create_alto (include/alto.hpp:108-294): the tensor is built on the device
+// from spt (which the caller may destroy afterwards); nprtn sets the
+// partition bounds and intervals the reference's OpenMP threads would use.
 template <typename LIT>
 static inline void create_alto(SparseTensor* spt, AltoTensor<LIT>** at, int nprtn)
 {
     using namespace alto_dropin;
+    static_assert(sizeof(LIT) == 8 || sizeof(LIT) == 16, "LIT must be a 64- or 128-bit unsigned integer");
     alto_tensor_t* t = nullptr;
-#ifdef VAL_TYPE_INT64
+#if defined(VAL_TYPE_INT64) || defined(VAL_TYPE_INT32)
     const int vk = ALTO_VAL_U64;
 #else
     const int vk = ALTO_VAL_F64;
 #endif
-    check(alto_tensor_create(spt->nmodes, (const uint64_t*)spt->dims, spt->nnz, (const uint64_t* const*)spt->cidx, spt->vals,
-                             ALTO_PTR_HOST, vk, lit_bits<LIT>(), nprtn, nullptr, &t),
+    check(alto_tensor_create(spt->nmodes, (const uint64_t*)spt->dims, spt->nnz, (const uint64_t* const*)spt->cidx,
+                             spt->vals, ALTO_PTR_HOST, vk, lit_bits<LIT>(), nprtn, nullptr, &t),
           "create_alto");
     alto_tensor_info_t inf;
     check(alto_tensor_info(t, &inf), "alto_tensor_info");
-    AltoTensor<LIT>* a = new AltoTensor<LIT>();
+    AltoTensor<LIT>* a = amalloc<AltoTensor<LIT>>(1);
+    memset((void*)a, 0, sizeof(*a));
     a->nmode = inf.nmodes;
     a->nprtn = inf.nprtn;
     a->nnz = inf.nnz;
-    a->dims = new IType[inf.nmodes];
-    a->mode_masks = new LIT[inf.nmodes];
-    a->cr_masks = new LIT[inf.nmodes];
+    a->dims = amalloc<IType>(inf.nmodes);
+    a->mode_masks = amalloc<LIT>(inf.nmodes);
+    a->cr_masks = amalloc<LIT>(inf.nmodes);
     for (int m = 0; m < inf.nmodes; ++m) {
         a->dims[m] = inf.dims[m];
         a->mode_masks[m] = lit_from<LIT>(&inf.mode_masks[m * inf.lit_words], inf.lit_words);
@@ -147,30 +125,60 @@ static inline void create_alto(SparseTensor* spt, AltoTensor<LIT>** at, int nprt
     }
     a->alto_mask = lit_from<LIT>(inf.alto_mask, inf.lit_words);
     a->alto_cr_mask = lit_from<LIT>(inf.alto_cr_mask, inf.lit_words);
-    a->is_oidx = inf.is_oidx != 0;
-    a->prtn_ptr = new IType[nprtn + 1];
+    a->prtn_ptr = amalloc<IType>(nprtn + 1);
     std::vector<uint64_t> iv((size_t)nprtn * inf.nmodes * 2);
+    a->prtn_intervals = amalloc<Interval>((size_t)nprtn * inf.nmodes);
+#ifndef ALTO_DROPIN_DEVICE_ONLY
+    std::vector<uint64_t> keys((size_t)inf.nnz * inf.lit_words);
+    std::vector<double> v(inf.nnz);
+    check(alto_tensor_export(t, keys.data(), v.data(), (uint64_t*)a->prtn_ptr, iv.data()), "alto_tensor_export");
+    a->idx = amalloc<LIT>(inf.nnz);
+    a->vals = amalloc<ValType>(inf.nnz);
+    for (IType i = 0; i < inf.nnz; ++i) {
+        a->idx[i] = lit_from<LIT>(&keys[(size_t)i * inf.lit_words], inf.lit_words);
+        a->vals[i] = (ValType)v[i];
+    }
+    a->is_oidx = inf.is_oidx != 0;
+    if (a->is_oidx) {
+        a->oidx = amalloc<IType*>(inf.nmodes);
+        for (int m = 0; m < inf.nmodes; ++m) {
+            a->oidx[m] = nullptr;
+            if (inf.nnz / inf.dims[m] <= MIN_FIBER_REUSE) {
+                a->oidx[m] = amalloc<IType>(inf.nnz);
+                check(alto_tensor_export_oidx(t, m, (uint64_t*)a->oidx[m]), "alto_tensor_export_oidx");
+            }
+        }
+    }
+#else
     check(alto_tensor_export(t, nullptr, nullptr, (uint64_t*)a->prtn_ptr, iv.data()), "alto_tensor_export");
-    a->prtn_intervals = new Interval[(size_t)nprtn * inf.nmodes];
+#endif
     for (size_t k = 0; k < (size_t)nprtn * inf.nmodes; ++k) a->prtn_intervals[k] = {iv[2 * k], iv[2 * k + 1]};
     a->dev = t;
     *at = a;
 }
 
+// destroy_alto (include/alto.hpp:296-322)
 template <typename LIT>
 static inline void destroy_alto(AltoTensor<LIT>* at)
 {
     alto_tensor_destroy(at->dev);
-    delete[] at->dims;
-    delete[] at->mode_masks;
-    delete[] at->cr_masks;
-    delete[] at->prtn_ptr;
-    delete[] at->prtn_intervals;
-    delete at;
+    AlignedFree(at->dims);
+    AlignedFree(at->mode_masks);
+    AlignedFree(at->cr_masks);
+    AlignedFree(at->idx);
+    AlignedFree(at->vals);
+    AlignedFree(at->prtn_ptr);
+    AlignedFree(at->prtn_intervals);
+    if (at->is_oidx && at->oidx) {
+        for (int n = 0; n < at->nmode; ++n) AlignedFree(at->oidx[n]);
+        AlignedFree(at->oidx);
+    }
+    AlignedFree(at);
 }
 
 // The device MTTKRP needs no per-partition fiber copies: create_da_mem /
-// destroy_da_mem keep their signatures (include/alto.hpp:324-398) and do nothing.
+// destroy_da_mem keep their signatures (include/alto.hpp:324-398) and do
+// nothing.
 template <typename LIT>
 static inline void create_da_mem(int, IType, AltoTensor<LIT>*, FType*** ofibs)
 {
@@ -181,7 +189,8 @@ static inline void destroy_da_mem(AltoTensor<LIT>*, FType**, IType, int)
 {
 }
 
-// mttkrp_alto_par (include/alto.hpp:617-623): factors[target_mode] += MTTKRP
+// mttkrp_alto_par (include/alto.hpp:617-623): factors[target_mode] += the
+// mode-target_mode MTTKRP of at with factors (host, dims[n] x rank)
 template <typename LIT>
 static inline void mttkrp_alto_par(int target_mode, FType** factors, IType rank, AltoTensor<LIT>* at, void* /*wlocks*/,
                                    FType** /*ofibs*/)
@@ -196,7 +205,7 @@ static inline void mttkrp_alto_par(int target_mode, FType** factors, IType rank,
           "alto_memcpy_d2h");
 }
 
-// mttkrp_alto (include/alto.hpp:457-543): same result
+// mttkrp_alto (include/alto.hpp:457-543): the same result
 template <typename LIT>
 void mttkrp_alto(int target_mode, FType** factors, IType rank, AltoTensor<LIT>* at)
 {
@@ -230,81 +239,6 @@ static inline void mttkrp_alto_batched(int njobs, const int* target_modes, FType
         const int m = target_modes[i];
         check(alto_memcpy_d2h(factors[i][m], d[i]->p[m], ats[i]->dims[m] * rank * sizeof(FType)), "alto_memcpy_d2h");
     }
-}
-
-// ---------------------------------------------------------------- Kruskal model
-static inline void CreateKruskalModel(int mode, IType* dims, IType rank, KruskalModel** M_)
-{
-    KruskalModel* M = new KruskalModel();
-    M->mode = mode;
-    M->rank = rank;
-    M->dims = new IType[mode];
-    memcpy(M->dims, dims, sizeof(IType) * mode);
-    M->U = new FType*[mode];
-    for (int n = 0; n < mode; ++n) M->U[n] = new FType[dims[n] * rank];
-    M->lambda = new FType[rank];
-    *M_ = M;
-}
-
-static inline void KruskalModelRandomInit(KruskalModel* M, unsigned int seed)
-{
-    alto_dropin::check(alto_kruskal_random_init(M->mode, (const uint64_t*)M->dims, M->rank, seed, M->U, M->lambda),
-                       "KruskalModelRandomInit");
-}
-
-static inline void DestroyKruskalModel(KruskalModel* M)
-{
-    for (int n = 0; n < M->mode; ++n) delete[] M->U[n];
-    delete[] M->U;
-    delete[] M->dims;
-    delete[] M->lambda;
-    delete M;
-}
-
-// update_gram (src/kruskal_model.cpp:11-24): both triangles are written
-static inline void update_gram(FType* gram, KruskalModel* M, int mode)
-{
-    using namespace alto_dropin;
-    void *u = nullptr, *g = nullptr;
-    const IType R = M->rank;
-    check(alto_device_malloc(&u, M->dims[mode] * R * sizeof(FType)), "alto_device_malloc");
-    check(alto_device_malloc(&g, R * R * sizeof(FType)), "alto_device_malloc");
-    check(alto_memcpy_h2d(u, M->U[mode], M->dims[mode] * R * sizeof(FType)), "h2d");
-    check(alto_update_gram(u, M->dims[mode], R, ALTO_PREC_F64, (double*)g, nullptr), "update_gram");
-    check(alto_memcpy_d2h(gram, g, R * R * sizeof(FType)), "d2h");
-    alto_device_free(u);
-    alto_device_free(g);
-}
-
-// ---------------------------------------------------------------- CP-ALS / CP-APR
-template <typename LIT>
-void cpd_alto(AltoTensor<LIT>* AT, KruskalModel* M, int max_iters, double epsilon)
-{
-    int iters = 0;
-    alto_cpd_timers_t tm;
-    alto_dropin::check(alto_cpd_als(AT->dev, M->rank, max_iters, epsilon, ALTO_PREC_F64, M->U, M->lambda, nullptr,
-                                    &iters, &tm, 1, nullptr),
-                       "cpd_alto");
-}
-
-// cpd_fit_alto (include/cpd.hpp:189-271): the fit of model M; the Grams and
-// the last-mode MTTKRP are recomputed on the device from M.
-template <typename LIT>
-double cpd_fit_alto(AltoTensor<LIT>* AT, KruskalModel* M, FType** /*grams*/, FType* /*U_mttkrp*/, FType /*normAT*/)
-{
-    double fit = 0;
-    alto_dropin::check(alto_cpd_fit(AT->dev, M->rank, M->U, M->lambda, &fit), "cpd_fit_alto");
-    return fit;
-}
-
-template <typename LIT>
-static inline void cp_apr_alto(AltoTensor<LIT>* AT, KruskalModel* M, int max_iters, int max_inner_iters,
-                               FType epsilon, FType eps_div_zero, FType kappa_tol, FType kappa, bool dobench)
-{
-    alto_apr_result_t res;
-    alto_dropin::check(alto_cp_apr(AT->dev, M->rank, max_iters, max_inner_iters, epsilon, eps_div_zero, kappa_tol,
-                                   kappa, dobench ? 1 : 0, M->U, M->lambda, &res, 1, nullptr),
-                       "cp_apr_alto");
 }
 
 #endif  // ALTO_DROPIN_ALTO_HPP_

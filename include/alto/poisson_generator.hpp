@@ -1,13 +1,15 @@
-// Synthetic Poisson tensors for the command line's -d / -s options: the
-// reference's generator (src/poisson_generator.cpp:336-376, called by
-// MakeSparseTensor, src/main.cpp:653-675), restated step by step so that a
-// run reproduces the reference's tensor for the same thread count T.
+// Synthetic Poisson tensors of the reference (include/poisson_generator.hpp,
+// src/poisson_generator.cpp:336-376, called by MakeSparseTensor,
+// src/main.cpp:653-675), restated step by step so that a run reproduces the
+// reference's tensor and model for the same thread count T.  Host code, as
+// there: it makes test data, not decompositions.
 //
 //  - T uniform streams, MT19937-64 seeded 1234 + t (src/rng.cpp is that
-//    published generator; std::mt19937_64 is the same algorithm and seeding);
-//    uniform draws ((x >> 12) + 0.5) 2^-52 scaled to [low, high)
-//    (include/rng.hpp:20, include/rng_stream.hpp:21-28); a batch of n draws
-//    is split into T chunks of ceil(n / T), chunk t from stream t
+//    published generator; std::mt19937_64 is the same algorithm and seeding),
+//    created by CreatePoissonGenerator and kept until
+//    DestroyPoissonGenerator; uniform draws ((x >> 12) + 0.5) 2^-52 scaled to
+//    [low, high) (include/rng.hpp:20, include/rng_stream.hpp:21-28); a batch
+//    of n draws is split into T chunks of ceil(n / T), chunk t from stream t
 //    (poisson_generator.cpp:20-34).
 //  - Model: every factor entry 0.1, then per column ceil(0.2 I_n) entries
 //    picked by a partial Fisher-Yates shuffle on glibc rand() get uniform
@@ -20,8 +22,10 @@
 //    coordinates merged into counts (:263-333).
 // The column sums are taken in row order, as one thread does; with T > 1
 // the reference's OpenMP reductions may round differently.
-#pragma once
+#ifndef ALTO_DROPIN_POISSON_GENERATOR_HPP_
+#define ALTO_DROPIN_POISSON_GENERATOR_HPP_
 
+#include <assert.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -29,15 +33,31 @@
 
 #include <algorithm>
 #include <random>
+#include <thread>
 #include <utility>
 #include <vector>
 
-#include "alto/alto.hpp"
+#include "common.hpp"
+#include "kruskal_model.hpp"
+#include "sptensor.hpp"
 
 namespace alto_gen {
 
 constexpr uint64_t kBatch = 100000;
 constexpr uint64_t kSeed = 1234;
+
+// the reference's stream count: the OpenMP thread count (OMP_NUM_THREADS,
+// else the hardware threads)
+inline int default_threads()
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    int t = (int)std::thread::hardware_concurrency();
+    if (const char* e = getenv("OMP_NUM_THREADS")) t = atoi(e);
+    return t < 1 ? 1 : t;
+#endif
+}
 
 struct Streams {
     std::vector<std::mt19937_64> s;
@@ -77,11 +97,13 @@ inline uint64_t binning(double rv, uint64_t nbins, const double* bins)
     return (uint64_t)hi;
 }
 
-// X with 0-based coordinates and integer counts; prints the reference's
-// progress lines
-inline SparseTensor* poisson_tensor(int nmodes, const IType* dims, IType num_edges, IType rank, int threads)
+// X with 0-based coordinates and integer counts, drawing from rng; the
+// model the edges were drawn from goes into *model when given (a model of
+// nmodes modes, these dims and this rank); prints the reference's progress
+// lines
+inline SparseTensor* poisson_tensor(int nmodes, const IType* dims, IType num_edges, IType rank, Streams& rng,
+                                    KruskalModel* model = nullptr)
 {
-    Streams rng(threads);
     IType maxd = 0;
     for (int n = 0; n < nmodes; ++n) maxd = dims[n] > maxd ? dims[n] : maxd;
     printf("Generating a sparse tensor: ");
@@ -121,6 +143,10 @@ inline SparseTensor* poisson_tensor(int nmodes, const IType* dims, IType num_edg
     for (IType j = 0; j < rank; ++j) tot += lambda[j];
     tot = 1.0 / tot;
     for (IType j = 0; j < rank; ++j) lambda[j] *= tot;
+    if (model) {
+        for (int n = 0; n < nmodes; ++n) memcpy(model->U[n], U[n].data(), dims[n] * rank * sizeof(FType));
+        memcpy(model->lambda, lambda.data(), rank * sizeof(FType));
+    }
 
     // the edges: counts per component, then each component's indices
     std::vector<double> bins(maxd + 1), work(kBatch);
@@ -169,10 +195,6 @@ inline SparseTensor* poisson_tensor(int nmodes, const IType* dims, IType num_edg
         }
         std::swap(src, dst);
     }
-    SparseTensor* X = new SparseTensor();
-    X->nmodes = nmodes;
-    X->dims = new IType[nmodes];
-    for (int n = 0; n < nmodes; ++n) X->dims[n] = dims[n];
     std::vector<IType> keep;
     std::vector<double> cnt;
     for (IType i = 0; i < ne; ++i) {
@@ -185,16 +207,72 @@ inline SparseTensor* poisson_tensor(int nmodes, const IType* dims, IType num_edg
             cnt.push_back(1.0);
         }
     }
-    X->nnz = keep.size();
-    X->cidx = new IType*[nmodes];
-    for (int n = 0; n < nmodes; ++n) {
-        X->cidx[n] = new IType[X->nnz];
+    SparseTensor* X = alto_io::alloc_tensor(nmodes, dims, keep.size());
+    for (int n = 0; n < nmodes; ++n)
         for (IType k = 0; k < X->nnz; ++k) X->cidx[n][k] = src[keep[k] * nmodes + n];
-    }
-    X->vals = new FType[X->nnz];
     for (IType k = 0; k < X->nnz; ++k) X->vals[k] = cnt[k];
     printf("    Nonzeros = %llu\n", (unsigned long long)X->nnz);
     return X;
 }
 
+// the same with T fresh streams
+inline SparseTensor* poisson_tensor(int nmodes, const IType* dims, IType num_edges, IType rank, int threads)
+{
+    Streams rng(threads);
+    return poisson_tensor(nmodes, dims, num_edges, rank, rng);
+}
+
+// the streams CreatePoissonGenerator makes (a global there too,
+// src/poisson_generator.cpp:17-18)
+inline Streams*& global_streams()
+{
+    static Streams* s = nullptr;
+    return s;
+}
+
 }  // namespace alto_gen
+
+// include/poisson_generator.hpp:9-13
+typedef struct PoissonGenerator_ {
+    int mode;
+    IType* dims;
+} PoissonGenerator;
+
+// src/poisson_generator.cpp:102-130: one MT19937-64 stream per OpenMP thread
+inline void CreatePoissonGenerator(int mode, IType* dims, PoissonGenerator** pg_)
+{
+    assert(mode > 0 && mode <= MAX_NUM_MODES);
+    PoissonGenerator* pg = (PoissonGenerator*)AlignedMalloc(sizeof(PoissonGenerator));
+    assert(pg);
+    pg->mode = mode;
+    pg->dims = (IType*)AlignedMalloc(mode * sizeof(IType));
+    memcpy(pg->dims, dims, mode * sizeof(IType));
+    delete alto_gen::global_streams();
+    alto_gen::global_streams() = new alto_gen::Streams(alto_gen::default_threads());
+    *pg_ = pg;
+}
+
+// src/poisson_generator.cpp:132-145
+inline void DestroyPoissonGenerator(PoissonGenerator* pg)
+{
+    delete alto_gen::global_streams();
+    alto_gen::global_streams() = nullptr;
+    AlignedFree(pg->dims);
+    AlignedFree(pg);
+}
+
+// src/poisson_generator.cpp:336-376: a random rank-`rank` model *M_ and
+// num_edges Poisson draws from it, merged into the count tensor *X_
+inline void PoissonGeneratorRun(PoissonGenerator* pg, IType num_edges, IType rank, KruskalModel** M_,
+                                SparseTensor** X_)
+{
+    for (int n = 0; n < pg->mode; n++) assert(pg->dims[n] >= rank);
+    assert(num_edges > 0);
+    assert(alto_gen::global_streams());
+    KruskalModel* M;
+    CreateKruskalModel(pg->mode, pg->dims, rank, &M);
+    *X_ = alto_gen::poisson_tensor(pg->mode, pg->dims, num_edges, rank, *alto_gen::global_streams(), M);
+    *M_ = M;
+}
+
+#endif  // ALTO_DROPIN_POISSON_GENERATOR_HPP_

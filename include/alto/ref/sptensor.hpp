@@ -1,0 +1,5 @@
+// Forwarding header: the reference include/sptensor.hpp name, so that code written
+// against the reference (for example its src/main.cpp) builds unchanged with
+// -I include/alto/ref and links libalto_hip.
+#pragma once
+#include "../sptensor.hpp"

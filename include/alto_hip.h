@@ -122,6 +122,19 @@ int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* mod
                         const void* const* const* factors, void* const* outs, int precision, int accumulate,
                         alto_stream_t stream);
 
+/* mttkrp / mttkrp_par on a coordinate tensor (include/cpd.hpp:460-543),
+ * without building an ALTO tensor: out (dims[mode] x rank, double) += the
+ * sum over the nnz nonzeros of vals[i] times the Hadamard product of rows
+ * cidx[m][i] of factors[m], m != mode (FP64, atomic adds in any order);
+ * accumulate == 0 zeroes out first.  factors[mode] is not read (it may be
+ * NULL or alias out).  cidx: nmodes arrays of zero-based coordinates.  All
+ * pointers of ptr_kind: host coordinates are range-checked (ALTO_ERR_ARG),
+ * device ones skip a nonzero with a coordinate out of range.  ALTO_PTR_HOST: synchronous;
+ * ALTO_PTR_DEVICE: asynchronous on `stream`. */
+int alto_coo_mttkrp(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64_t* const* cidx,
+                    const double* vals, int mode, uint64_t rank, const double* const* factors, double* out,
+                    int accumulate, int ptr_kind, alto_stream_t stream);
+
 /* Tuning knobs of the stream kernels (no reference counterpart; the
  * reference specializes at compile time, config.mk:25-26).  ALTO_TUNE_SEG:
  * nonzeros walked by one lane group (0 = default).  ALTO_TUNE_CHUNK_LOG2:
@@ -149,7 +162,9 @@ int alto_mttkrp_batched(int njobs, alto_tensor_t* const* tensors, const int* mod
  * default 32) use two columns per lane in the vector-lane mapping (fewer
  * lane groups per wave, so fewer rounds of FP64 row flushes).
  * ALTO_TUNE_PHI_VARIANT: CP-APR Phi kernel, 0 = runtime batch loop with LDS
- * permutes, 1 = compile-time batch with DPP broadcasts (default).
+ * permutes, 1 = compile-time batch with DPP broadcasts (default), 2 = 16-byte
+ * lanes with the model row reloaded only where the row changes (even ranks
+ * and 16-byte aligned factors; others take 1).
  * ALTO_TUNE_NARROW_SMALL: 1 (default) = below 4M nonzeros every FP32 / BF16
  * rank uses two columns per lane.
  * ALTO_TUNE_FIBERS: 1 (default) = mode streams of tensors with at least
@@ -244,6 +259,16 @@ int alto_cpd_session_destroy(alto_cpd_session_t* s);
 int alto_cpd_fit(alto_tensor_t* t, uint64_t rank, double* const* U, const double* lambda,
                  double* fit);
 
+/* cpd_fit_alto (include/cpd.hpp:189-271) from its arguments: the fit of the
+ * model (lambda, last factor U_last: dims[nmodes-1] x rank) against a tensor
+ * of squared norm normX, given the Grams (nmodes rank x rank matrices, only
+ * the lower triangle read, as the reference defines it) and the last mode's
+ * MTTKRP U_mttkrp.  Pointers of ptr_kind; the arithmetic runs on the
+ * device. */
+int alto_cpd_fit_parts(int nmodes, const uint64_t* dims, uint64_t rank, const double* const* grams,
+                       const double* lambda, const double* U_last, const double* U_mttkrp, double normX, int ptr_kind,
+                       double* fit);
+
 /* pseudo_inverse (include/cpd.hpp:546-646): U (device dim x rank, double)
  * <- U V^-1 with V the Hadamard product of grams[m != mode] (device, nmodes
  * consecutive rank x rank matrices).  Cholesky; when it fails, the minimum-
@@ -287,11 +312,35 @@ int alto_cp_apr(alto_tensor_t* t, uint64_t rank, int max_iters, int max_inner, d
                 double eps_div_zero, double kappa_tol, double kappa, int dobench, double* const* U,
                 double* lambda, alto_apr_result_t* res, int verbose, alto_stream_t stream);
 
+/* cp_apr_alto with a choice of stop rule: ALTO_APR_STOP_ALTO (cp_apr_alto's
+ * |KKT - previous KKT| <= epsilon^2, include/apr.hpp:1261), ALTO_APR_BENCH
+ * (its dobench: every iteration runs), ALTO_APR_STOP_COO (the COO driver
+ * cp_apr_mu_alto / cp_apr_mu_alto_otf, src/apr.cpp:239-779: stop after an
+ * outer iteration in which every mode's first inner step met epsilon; that
+ * iteration counts).  Otherwise as alto_cp_apr. */
+enum { ALTO_APR_STOP_ALTO = 0, ALTO_APR_BENCH = 1, ALTO_APR_STOP_COO = 2 };
+int alto_cp_apr_ex(alto_tensor_t* t, uint64_t rank, int max_iters, int max_inner, double epsilon,
+                   double eps_div_zero, double kappa_tol, double kappa, int stop_rule, double* const* U,
+                   double* lambda, alto_apr_result_t* res, int verbose, alto_stream_t stream);
+
 /* CalculatePhi_alto_par (include/apr.hpp:295-303): Phi (device, I_mode x rank,
  * double) = sum over the mode's nonzeros of (x / max(<B[row,:], pi>, eps)) * pi,
  * B = factors[mode].  factors: nmodes device pointers (double). */
 int alto_phi(alto_tensor_t* t, int mode, uint64_t rank, const double* const* factors, double* phi,
              double eps_div_zero, alto_stream_t stream);
+
+/* Kruskal model utilities on the device (pointers of ptr_kind):
+ * ALTO_KRUSKAL_L1 -- KruskalModelNormalizeMode (src/kruskal_model.cpp:178-222)
+ *   on `mode`, or KruskalModelNormalize (:128-176) on every mode when mode
+ *   is -1: columns divided by their L1 norms, lambda multiplied by them;
+ * ALTO_KRUSKAL_NORM2 / ALTO_KRUSKAL_NORMMAX -- KruskalModelNorm (:307-341)
+ *   MAT_NORM_2 / MAT_NORM_MAX on `mode`: lambda = the columns' 2-norms, or
+ *   max(1, largest entry) (signed), columns divided by lambda;
+ * ALTO_KRUSKAL_REDISTRIBUTE -- RedistributeLambda (:355-378) on `mode`:
+ *   columns multiplied by lambda, lambda = 1.  Synchronous. */
+enum { ALTO_KRUSKAL_L1 = 0, ALTO_KRUSKAL_NORM2 = 1, ALTO_KRUSKAL_NORMMAX = 2, ALTO_KRUSKAL_REDISTRIBUTE = 3 };
+int alto_kruskal_normalize(int nmodes, const uint64_t* dims, uint64_t rank, double* const* U, double* lambda,
+                           int mode, int kind, int ptr_kind, alto_stream_t stream);
 
 /* KruskalModelRandomInit (src/kruskal_model.cpp:109-125) on host buffers:
  * glibc srand(seed) then U[n][i] = rand()/RAND_MAX row-major, lambda = 1. */
@@ -349,6 +398,8 @@ int alto_device_malloc(void** p, uint64_t bytes);
 int alto_device_free(void* p);
 int alto_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 int alto_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+/* device to device, asynchronous on `stream` */
+int alto_memcpy_d2d(void* dst, const void* src, uint64_t bytes, alto_stream_t stream);
 int alto_device_synchronize(void);
 /* Returns the library's cached device blocks to the runtime (the library
  * reuses freed buffers internally so that concurrent work never blocks on

@@ -24,12 +24,17 @@ f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 
 def build(with_ref: bool | None = None) -> None:
-    """Compile liboracle.so (and _ref/ when the reference tree is present)."""
+    """Compile liboracle.so and, when the reference tree is present, _ref/:
+    the reference's format + MTTKRP path, and its command line (src/main.cpp)
+    built against the drop-in headers and linked with libalto_hip (when that
+    library has been built)."""
     subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
     if with_ref is None:
         with_ref = os.path.isdir("/root/reference/include")
     if with_ref:
         subprocess.check_call(["make", "-s", "-C", _HERE, "ref"])
+        if os.path.exists(os.path.join(os.path.dirname(_HERE), "alto_amd", "libalto_hip.so")):
+            subprocess.check_call(["make", "-s", "-C", _HERE, "dropin"])
 
 
 APR_HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p),

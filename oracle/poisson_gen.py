@@ -87,8 +87,10 @@ def binning(rv, bins):
     return end
 
 
-def poisson_tensor(dims, num_edges, rank, threads=1):
-    """PoissonGeneratorRun (:336-376) -> (cidx: list of per-mode lists, vals)."""
+def poisson_tensor(dims, num_edges, rank, threads=1, return_model=False):
+    """PoissonGeneratorRun (:336-376) -> (cidx: list of per-mode lists, vals),
+    and the model the edges were drawn from, (factors, lambda) as float64
+    arrays, when return_model is set."""
     libc = ctypes.CDLL("libc.so.6")
     libc.srand(1)
     rng = Streams(threads)
@@ -151,4 +153,10 @@ def poisson_tensor(dims, num_edges, rank, threads=1):
         else:
             cidx.append(e)
             vals.append(1.0)
-    return [[c[n] for c in cidx] for n in range(N)], vals
+    out = ([[c[n] for c in cidx] for n in range(N)], vals)
+    if return_model:
+        import numpy as np
+        model = ([np.array(U[n], np.float64).reshape(d, rank) for n, d in enumerate(dims)],
+                 np.array(lam, np.float64))
+        return out + (model,)
+    return out

@@ -56,3 +56,40 @@ def test_invalid_arguments_rejected_without_gpu_work():
     assert L.alto_cpd_sessions_iterate(1, one, -1, 0.0, None, None) < 0
     assert L.alto_cpd_sessions_iterate(0, None, 5, 0.0, None, None) == A.ALTO_OK
     assert L.alto_version().startswith(b"alto-mi355x")
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libalto_hip.so not built")
+def test_dropin_entry_points_reject_bad_arguments_on_the_host():
+    """The entry points behind the C++ drop-in (COO MTTKRP, Kruskal
+    normalisation, fit from parts, CP-APR with a stop rule) reject invalid
+    arguments before any device work; a host coordinate out of range is
+    found by the range check."""
+    import numpy as np
+
+    import alto_amd as A
+
+    L = A.lib()
+    dims = (ctypes.c_uint64 * 3)(4, 5, 6)
+    U = [np.zeros((d, 2)) for d in (4, 5, 6)]
+    up = (ctypes.c_void_p * 3)(*[u.ctypes.data for u in U])
+    cidx = np.array([[0, 3], [1, 4], [2, 6]], np.uint64)  # 6 is out of range in mode 2
+    cp = (ctypes.c_void_p * 3)(*[cidx[m].ctypes.data for m in range(3)])
+    vals = np.ones(2)
+    out = np.zeros((4, 2))
+    assert L.alto_coo_mttkrp(0, dims, 2, cp, vals.ctypes.data, 0, 2, up, out.ctypes.data, 1, 0, None) < 0
+    assert L.alto_coo_mttkrp(3, dims, 2, cp, vals.ctypes.data, 3, 2, up, out.ctypes.data, 1, 0, None) < 0
+    assert L.alto_coo_mttkrp(3, dims, 2, cp, vals.ctypes.data, 0, 0, up, out.ctypes.data, 1, 0, None) < 0
+    assert L.alto_coo_mttkrp(3, dims, 2, cp, vals.ctypes.data, 0, 2, up, out.ctypes.data, 1, 7, None) < 0
+    assert L.alto_coo_mttkrp(3, dims, 2, cp, vals.ctypes.data, 0, 2, up, out.ctypes.data, 1, 0, None) < 0
+    assert b"out of range" in L.alto_last_error()
+    lam = np.ones(2)
+    assert L.alto_kruskal_normalize(3, dims, 0, up, lam.ctypes.data, 0, 0, 0, None) < 0
+    assert L.alto_kruskal_normalize(3, dims, 2, up, lam.ctypes.data, -1, A.KRUSKAL_NORM2, 0, None) < 0
+    assert L.alto_kruskal_normalize(3, dims, 2, up, lam.ctypes.data, 3, A.KRUSKAL_L1, 0, None) < 0
+    assert L.alto_kruskal_normalize(3, dims, 2, up, lam.ctypes.data, 0, 9, 0, None) < 0
+    fit = ctypes.c_double()
+    assert L.alto_cpd_fit_parts(3, dims, 2, None, lam.ctypes.data, up[2], up[2], 1.0, 0, ctypes.byref(fit)) < 0
+    assert L.alto_cpd_fit_parts(0, dims, 2, up, lam.ctypes.data, up[2], up[2], 1.0, 0, ctypes.byref(fit)) < 0
+    res = A.AprResult()
+    assert L.alto_cp_apr_ex(None, 2, 1, 1, 0.0, 0.0, 0.0, 0.0, A.APR_STOP_COO, up, lam.ctypes.data,
+                            ctypes.byref(res), 0, None) < 0

@@ -18,6 +18,8 @@ def test_dropin_header_compiles(tmp_path):
     src = tmp_path / "use.cpp"
     src.write_text('''
 #include "alto/alto.hpp"
+#include "alto/apr.hpp"
+#include "alto/cpd.hpp"
 int main() {
     IType dims[3] = {4, 8, 2};
     KruskalModel* M; CreateKruskalModel(3, dims, 2, &M);
@@ -61,9 +63,22 @@ def test_cli_check_bench_and_run(tmp_path):
     assert r.returncode == 0, r.stderr
     its = [l for l in r.stdout.splitlines() if l.startswith("it: ")]
     assert len(its) == 6 and "Per iteration" in r.stdout
+    # the fits printed (%g, 6 significant digits) are the oracle's CP-ALS fits
+    import oracle
+    cidx, vals = uniform_tensor([30, 40, 50], 4000, 3)
+    o = oracle.OracleAlto([30, 40, 50], cidx, vals, nprtn=16)
+    U, lam = oracle.random_init([30, 40, 50], 8, 1)
+    _, fits = o.cpd_als(U, lam, 6, 1e-12)
+    got = [float(l.split("\t")[1].split()[1]) for l in its]
+    assert all(abs(g - f) <= 5e-6 * max(1.0, abs(f)) for g, f in zip(got, fits)), (got, fits)
     r = subprocess.run([os.path.join(BIN, "cpd64"), "-l", "APR", "-i", str(tns), "-r", "4", "-m", "3", "-n", "2",
                         "-x", "1"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "Final log-likelihood" in r.stdout, r.stdout + r.stderr
+    U, lam = oracle.random_init([30, 40, 50], 4, 1)
+    res = o.cp_apr(U, lam, 3, 2, eps=1e-5)
+    line = [l for l in r.stdout.splitlines() if "Final log-likelihood" in l][0]
+    assert abs(float(line.split("=")[1]) - res["loglik"]) <= 1e-6 * abs(res["loglik"]), (line, res)
+    assert f"Total inner iterations = {res['total_inner']}" in r.stdout
     # -f: the text input is also written as <N>D_<nnz>.bin; running from that
     # binary file gives the same fit trace
     runs = []
@@ -88,17 +103,18 @@ def test_cli_check_bench_and_run(tmp_path):
 
 
 _IO_PROG = r'''
-#include "tensor_io.hpp"
+#include "alto/sptensor.hpp"
 // t/z: read .tns (one-/zero-based); b: read binary; w: .tns -> binary under
 // the export name; x: binary -> .tns under the export name.  Prints the
 // tensor (nmodes nnz dims, then one line per nonzero) or the written name.
 int main(int argc, char** argv) {
     const char k = argv[1][0];
-    SparseTensor* X = (k == 'b' || k == 'x') ? read_bin(argv[2]) : read_tns(argv[2], k == 'z');
+    SparseTensor* X;
+    ImportSparseTensor(argv[2], (k == 'b' || k == 'x') ? BINARY_FORMAT : TEXT_FORMAT, &X, k == 'z');
     if (k == 'w' || k == 'x') {
-        const std::string fn = export_name(X, k == 'w');
-        write_tensor(fn.c_str(), X, k == 'w');
-        printf("%s\n", fn.c_str());
+        ExportSparseTensor(NULL, k == 'w' ? BINARY_FORMAT : TEXT_FORMAT, X);
+        printf("%s\n", alto_io::export_name(X, k == 'w').c_str());
+        DestroySparseTensor(X);
         return 0;
     }
     printf("%d %llu", X->nmodes, (unsigned long long)X->nnz);
@@ -108,6 +124,7 @@ int main(int argc, char** argv) {
         for (int m = 0; m < X->nmodes; ++m) printf("%llu ", (unsigned long long)X->cidx[m][i]);
         printf("%.17g\n", X->vals[i]);
     }
+    DestroySparseTensor(X);
     return 0;
 }
 '''
@@ -118,7 +135,7 @@ def _io_tool(tmp_path):
     src.write_text(_IO_PROG)
     exe = tmp_path / "io"
     r = subprocess.run(["g++", "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"),
-                        "-I" + os.path.join(ROOT, "alto_amd", "cli"), str(src), "-o", str(exe)],
+                        str(src), "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
@@ -172,15 +189,15 @@ def test_tensor_io_formats(tmp_path):
 
 
 _GEN_PROG = r'''
-#include "poisson_gen.hpp"
-#include "tensor_io.hpp"
+#include "alto/poisson_generator.hpp"
 // edges rank threads dims...: the -d / -s generator, written as g.bin
 int main(int argc, char** argv) {
     std::vector<IType> dims;
     for (int i = 4; i < argc; ++i) dims.push_back(atoll(argv[i]));
     SparseTensor* X = alto_gen::poisson_tensor((int)dims.size(), dims.data(), atoll(argv[1]), atoll(argv[2]),
                                                atoi(argv[3]));
-    write_tensor("g.bin", X, true);
+    ExportSparseTensor("g.bin", BINARY_FORMAT, X);
+    DestroySparseTensor(X);
     return 0;
 }
 '''
@@ -210,7 +227,7 @@ def test_poisson_generator_matches_oracle(tmp_path, dims, edges, rank, threads):
     src.write_text(_GEN_PROG)
     exe = tmp_path / "g"
     r = subprocess.run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"),
-                        "-I" + os.path.join(ROOT, "alto_amd", "cli"), str(src), "-o", str(exe)],
+                        str(src), "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([str(exe), str(edges), str(rank), str(threads)] + [str(d) for d in dims], cwd=tmp_path,
