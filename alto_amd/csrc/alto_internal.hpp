@@ -144,6 +144,9 @@ struct alto_tensor {
     // factor rows permuted into rank order for the stream kernels, one buffer
     // per mode (kept, grown on demand)
     alto::DevBuf pcopy[ALTO_MAXN];
+    // per-segment row slots of the deterministic-order walk (kept, grown on
+    // demand): the first / last row of each segment and their partial sums
+    alto::DevBuf det_rows, det_part;
 
     // MTTKRP launch shapes chosen by alto_mttkrp_autotune for (mode, rank,
     // precision); used when the process-wide knobs are at their defaults
@@ -182,6 +185,12 @@ int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s);
 // gets the layout
 int fiber_settings(int* min_len, int* chunk_log2, uint64_t* min_nnz);
 int fiber_rule();
+// deterministic-order reductions (ALTO_TUNE_DETERMINISTIC): streams fully
+// sorted by row, MTTKRP / Phi sums in a fixed order
+int det_enabled();
+int apr_pi_cache_enabled();
+int mttkrp_det_device(alto_tensor* t, int mode, int R, const void* const* U, double* out, int precision, hipStream_t s);
+int phi_det_device(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s);
 
 // cpd.hip: per-phase event timing of a session's eager iterations (default on)
 int cpd_session_set_timing(alto_cpd_session_t* sess, int on);

@@ -195,6 +195,7 @@ struct AprState {
     int N = 0, R = 0;
     DevBuf U[ALTO_MAXN], Phi[ALTO_MAXN];
     DevBuf lambda, ones, norm, cpart, kpart, kval, any, llpart, llval, grams, gpart, Mlast, ipart, fitbuf, nx, nxpart;
+    DevBuf pic;  // products of the other modes' rows per nonzero, for the inner iterations of one mode
 };
 
 static double now_s()
@@ -334,6 +335,17 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
     }
     const double* Uptr[ALTO_MAXN];
     for (int m = 0; m < N; ++m) Uptr[m] = c.U[m].as<double>();
+    // the inner iterations of a mode keep the other factors: their row
+    // products are stored per nonzero by the first Phi of the mode and read
+    // back by the others, when nnz x R doubles fit in half the free memory
+    // (at most 64 GiB)
+    bool cache = apr_pi_cache_enabled() && !det_enabled() && max_inner > 1;
+    if (cache) {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(hipMemGetInfo(&fr, &tot));
+        const uint64_t want = t->nnz * (uint64_t)R * sizeof(double);
+        cache = want <= fr / 2 && want <= (64ull << 30) && c.pic.alloc(want) == ALTO_OK;
+    }
     int iter;
     for (iter = 0; iter < max_iters; ++iter) {
         bool converged = coo;
@@ -357,7 +369,8 @@ int cp_apr_impl(alto_tensor* t, int R, int max_iters, int max_inner, double epsi
                 n_inner[iter]++;
                 inner_per_mode[n]++;
                 HIP_TRY(hipEventRecord(ev0[inner], s));
-                ALTO_TRY(phi_device(t, n, R, Uptr, Ph, eps_div_zero, s, 0));
+                ALTO_TRY(phi_device_cached(t, n, R, Uptr, Ph, eps_div_zero, s, 0, c.pic.as<double>(),
+                                           cache ? (inner == 0 ? 1 : 2) : 0));
                 HIP_TRY(hipEventRecord(ev1[inner], s));
                 timed = inner + 1;
                 const unsigned nb = ew_blocks(ne);

@@ -1098,6 +1098,10 @@ struct StreamLayout {
 static StreamLayout wanted_layout(const alto_tensor* t, int m, int ranked)
 {
     StreamLayout L{t->chunk_log2, -1, 0};
+    if (det_enabled()) {  // one chunk: each row's nonzeros contiguous
+        L.chunk_log2 = 63;
+        return L;
+    }
     int fmin = 0, fchunk = 0;
     uint64_t fnnz = 0;
     if (ranked || !fiber_settings(&fmin, &fchunk, &fnnz) || t->nmode < 3 || t->nnz < fnnz) return L;
@@ -1161,7 +1165,7 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     uint64_t n = t->nnz;
     const int want_chunk = tuned_chunk_log2();
     if (want_chunk > 0 && want_chunk != t->chunk_log2) t->chunk_log2 = want_chunk;
-    const int ranked = (int)rank_applicable(t);
+    const int ranked = det_enabled() ? 0 : (int)rank_applicable(t);
     StreamLayout want[ALTO_MAXN];
     bool any = false, reorder = false;
     for (int m = 0; m < t->nmode; ++m) {

@@ -273,6 +273,8 @@ inline int fill_zero(T* p, uint64_t n, hipStream_t s) { return fill_value<T>(p, 
 int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* out, int precision, int accumulate,
                   hipStream_t s, int seg);
 int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s, int seg);
+int phi_device_cached(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s,
+                      int seg, double* pc, int pim);
 int mttkrp_batched_impl(int njobs, alto_tensor_t* const* tensors, const int* modes, uint64_t rank,
                         const void* const* const* factors, void* const* outs, int precision, int accumulate,
                         hipStream_t s);

@@ -77,6 +77,13 @@ int fiber_settings(int* min_len, int* chunk_log2, uint64_t* min_nnz)
     *min_nnz = g_fiber_min_nnz;
     return g_fibers;
 }
+// deterministic-order MTTKRP / Phi (fixed summation order, no float atomics)
+static int g_det = 0;
+int det_enabled() { return g_det; }
+// CP-APR: products of the other modes' rows kept per nonzero across a
+// mode's inner iterations (when they fit)
+static int g_apr_pi_cache = 1;
+int apr_pi_cache_enabled() { return g_apr_pi_cache; }
 int graphs_enabled() { return g_graphs; }
 int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
@@ -192,9 +199,15 @@ __global__ void __launch_bounds__(256) k_phi(StreamArgs<double, double, double> 
 // raise the register count past the point where occupancy pays for them;
 // holding this kernel to 128 VGPRs for a fourth wave per SIMD measured
 // slower too).
-template <int G, int E, int PW, int NO>
+// PIM: the products pi of the other modes' rows are computed from the
+// factors (0), computed and stored per nonzero into Pc in stream order (1),
+// or read back from Pc (2): within CP-APR's inner iterations of one mode the
+// other factors do not change, so the later iterations stream pi instead of
+// gathering NO rows per nonzero (the reference's precomputed rows,
+// include/apr.hpp:1049-1097 and 1128-1165).
+template <int G, int E, int PW, int NO, int PIM>
 __global__ void __launch_bounds__(256)
-    k_phi_b(StreamArgs<double, double, double> a, const double* __restrict__ B, double eps)
+    k_phi_b(StreamArgs<double, double, double> a, const double* __restrict__ B, double eps, double* __restrict__ Pc)
 {
     constexpr int J = G < 4 ? G : 4;
     const int lane = threadIdx.x & 63;
@@ -242,18 +255,40 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                 for (int q = 0; q < E; ++q) pi[jj][q] = 1.0;
             });
-            static_for<0, NO>([&](auto oi) {
-                constexpr int o = decltype(oi)::value;
-                const double* __restrict__ U = a.Uo[o];
-                static_for<0, J>([&](auto ji) {
-                    constexpr int jj = decltype(ji)::value;
-                    const uint32_t off = bcast32<G, j0 + jj>(offs[o]);
-                    double x[E];
-                    load_cols<double, double, G, E>(U + off, gl, R, x);
+            if constexpr (PIM == 2) {
 #pragma unroll
-                    for (int q = 0; q < E; ++q) pi[jj][q] *= x[q];
+                for (int jj = 0; jj < J; ++jj) {
+                    const uint64_t i = p + j0 + jj < e ? p + j0 + jj : e - 1;
+#pragma unroll
+                    for (int q = 0; q < E; ++q) {
+                        const int c = gl + q * G;
+                        pi[jj][q] = c < R ? Pc[i * R + c] : 0.0;
+                    }
+                }
+            } else {
+                static_for<0, NO>([&](auto oi) {
+                    constexpr int o = decltype(oi)::value;
+                    const double* __restrict__ U = a.Uo[o];
+                    static_for<0, J>([&](auto ji) {
+                        constexpr int jj = decltype(ji)::value;
+                        const uint32_t off = bcast32<G, j0 + jj>(offs[o]);
+                        double x[E];
+                        load_cols<double, double, G, E>(U + off, gl, R, x);
+#pragma unroll
+                        for (int q = 0; q < E; ++q) pi[jj][q] *= x[q];
+                    });
                 });
-            });
+            }
+            if constexpr (PIM == 1) {
+#pragma unroll
+                for (int jj = 0; jj < J; ++jj)
+                    if (j0 + jj < cnt)
+#pragma unroll
+                        for (int q = 0; q < E; ++q) {
+                            const int c = gl + q * G;
+                            if (c < R) Pc[(p + j0 + jj) * R + c] = pi[jj][q];
+                        }
+            }
             if constexpr (j0 == 0) {
                 const uint64_t i = p + G + gl;
                 vn = 0;
@@ -494,6 +529,10 @@ static int mttkrp_device_shape(alto_tensor* t, int mode, int R, const void* cons
     }
     ALTO_TRY(mttkrp_prepare(t, R, precision, s));
     uint64_t nout = t->dims[mode] * (uint64_t)R;
+    if (g_det) {
+        if (!accumulate) ALTO_TRY(fill_zero((double*)out, nout, s));
+        return mttkrp_det_device(t, mode, R, U, (double*)out, precision, s);
+    }
     MttkrpTune k;
     k.variant = variant >= -1 ? variant : g_variant;
     if (variant == -2 && seg <= 0 && g_seg == 0 && g_variant < 0) {  // shape from the autotuner, if it ran
@@ -536,6 +575,15 @@ int mttkrp_device(alto_tensor* t, int mode, int R, const void* const* U, void* o
 
 int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s, int seg)
 {
+    return phi_device_cached(t, mode, R, U, phi, eps, s, seg, nullptr, 0);
+}
+
+// pim: 0 = no cache; 1 = store the products of the other modes' rows per
+// nonzero into pc (nnz x R doubles, stream order); 2 = read them from pc
+// (stored by a pim 1 call for the same mode and factors)
+int phi_device_cached(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s,
+                      int seg, double* pc, int pim)
+{
     if (mode < 0 || mode >= t->nmode || R < 1 || R > 128) {
         set_error("phi: mode %d / rank %d out of range (rank 1..128)", mode, R);
         return ALTO_ERR_ARG;
@@ -545,6 +593,7 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
     if (seg <= 0) seg = tuned_seg(t->nnz, group_lanes(R), 8);
     ALTO_TRY(fill_zero(phi, t->dims[mode] * (uint64_t)R, s));
     ALTO_TRY(reserve_rank_copies(t, (uint64_t)R, 8, s));
+    if (g_det) return phi_det_device(t, mode, R, U, phi, eps, s);
     StreamArgs<double, double, double> a;
     fill_stream_args(t, mode, R, seg, a);
     ALTO_TRY(set_factors(t, mode, U, a, s));
@@ -553,7 +602,7 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
     dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
     bool al16 = R % 2 == 0 && ((uintptr_t)U[mode] & 15) == 0;
     for (int o = 0; o < a.nother; ++o) al16 = al16 && ((uintptr_t)a.Uo[o] & 15) == 0;
-    if (g_phi_variant == 2 && al16) {
+    if (g_phi_variant == 2 && al16 && !pim) {
         const int no = t->nmode - 1;
 #define PHI16_NO_(N_) hipLaunchKernelGGL((k_phi16<G, PW, N_>), grid, dim3(256), 0, s, a, U[mode], eps)
         RANK_DISPATCH(R, t->pack.pw, {
@@ -563,9 +612,15 @@ int phi_device(alto_tensor* t, int mode, int R, const double* const* U, double* 
             else PHI16_NO_(4);
         });
 #undef PHI16_NO_
-    } else if (g_phi_variant >= 1) {
+    } else if (g_phi_variant >= 1 || pim) {
         const int no = t->nmode - 1;
-#define PHI_NO_(N_) hipLaunchKernelGGL((k_phi_b<G, E, PW, N_>), grid, dim3(256), 0, s, a, U[mode], eps)
+#define PHI_NO_(N_)                                                                                              \
+    do {                                                                                                         \
+        if (pim == 1) hipLaunchKernelGGL((k_phi_b<G, E, PW, N_, 1>), grid, dim3(256), 0, s, a, U[mode], eps, pc); \
+        else if (pim == 2)                                                                                       \
+            hipLaunchKernelGGL((k_phi_b<G, E, PW, N_, 2>), grid, dim3(256), 0, s, a, U[mode], eps, pc);          \
+        else hipLaunchKernelGGL((k_phi_b<G, E, PW, N_, 0>), grid, dim3(256), 0, s, a, U[mode], eps, pc);          \
+    } while (0)
         RANK_DISPATCH(R, t->pack.pw, {
             if (no == 1) PHI_NO_(1);
             else if (no == 2) PHI_NO_(2);
@@ -630,6 +685,14 @@ int alto_tune_set(int param, int value)
     }
     if (param == ALTO_TUNE_FIBERS) {
         g_fibers = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_DETERMINISTIC) {
+        g_det = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_APR_PI_CACHE) {
+        g_apr_pi_cache = value ? 1 : 0;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_FIBER_MIN) {
@@ -790,6 +853,11 @@ int mttkrp_batched_impl(int njobs, alto_tensor_t* const* tensors, const int* mod
                 set_error("alto_mttkrp_batched: job %d: null factor %d", i, m);
                 return ALTO_ERR_ARG;
             }
+    }
+    if (g_det) {  // one deterministic MTTKRP per job
+        for (int i = 0; i < njobs; ++i)
+            ALTO_TRY(mttkrp_device(tensors[i], modes[i], R, factors[i], outs[i], precision, accumulate, s, 0));
+        return ALTO_OK;
     }
     MttkrpTune k;
     k.variant = g_variant;

@@ -177,6 +177,9 @@ def main():
                     help="profiling driver: only the 16-job batched MTTKRP comparison, its JSON on stdout")
     ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the Poisson(50/s) serving stream to run (0 = skip; the full stream is 2000)")
+    ap.add_argument("--tune", action="append", default=[], metavar="PARAM=VALUE",
+                    help="alto_tune_set(PARAM, VALUE) before the run (alto_hip.h ALTO_TUNE_*), e.g. 13=0 for "
+                         "streams without the fiber layout; repeatable")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported (the multi-GPU "
@@ -189,6 +192,9 @@ def main():
     from alto_amd import gen
 
     A.lib()
+    for kv in args.tune:
+        k, v = kv.split("=")
+        A.check(A.lib().alto_tune_set(int(k), int(v)), f"alto_tune_set({kv})")
     stream = torch.cuda.current_stream().cuda_stream
     nnz = args.nnz
     if args.copy_only:

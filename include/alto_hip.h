@@ -175,12 +175,27 @@ int alto_coo_mttkrp(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64
  * (default 4) nonzeros first, walked by a kernel that reads the fiber-mode
  * factor row once per fiber, then the rest.  ALTO_TUNE_FIBER_MODE_RULE: 0 =
  * the fiber mode is the other mode with the fewest rows, 1 = with the most.
+ * ALTO_TUNE_DETERMINISTIC: 1 = MTTKRP (every precision, batched calls
+ * included) and Phi sum in one fixed order, with no floating-point atomics:
+ * the mode streams are fully sorted by row, each segment of 1024 nonzeros
+ * is walked in order, rows inside a segment are written by it alone and the
+ * rows split between segments are summed in segment order by a second
+ * launch.  Results are then bit-identical from run to run (slower: for
+ * parity runs).  Calls in this mode on one tensor share its slot buffer:
+ * they must not overlap on different streams.
+ * ALTO_TUNE_APR_PI_CACHE: 1 (default) = CP-APR stores, at the first inner
+ * iteration of each mode, the product of the other modes' factor rows for
+ * every nonzero (nnz x rank doubles, when that fits in half the free device
+ * memory and 64 GiB) and streams it in the mode's later inner iterations
+ * instead of gathering those rows again (the reference's precomputed rows,
+ * include/apr.hpp:1049-1165); 0 = gather them every time.
  * Streams are re-ordered on their next use. */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
        ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9,
        ALTO_TUNE_NARROW_RANK = 10, ALTO_TUNE_PHI_VARIANT = 11, ALTO_TUNE_NARROW_SMALL = 12,
        ALTO_TUNE_FIBERS = 13, ALTO_TUNE_FIBER_MIN = 14, ALTO_TUNE_FIBER_CHUNK_LOG2 = 15,
-       ALTO_TUNE_FIBER_MIN_NNZ = 16, ALTO_TUNE_FIBER_MODE_RULE = 17 };
+       ALTO_TUNE_FIBER_MIN_NNZ = 16, ALTO_TUNE_FIBER_MODE_RULE = 17, ALTO_TUNE_DETERMINISTIC = 18,
+       ALTO_TUNE_APR_PI_CACHE = 19 };
 int alto_tune_set(int param, int value);
 
 /* Autotuner (no reference counterpart; the reference fixes its shapes at

@@ -40,3 +40,29 @@ def test_cp_apr_matches_oracle(dims, nnz, R, W, bench):
     np.testing.assert_allclose(lam, lo, rtol=1e-9)
     for a, b in zip(U, Uo):
         np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-12)
+
+
+def test_cp_apr_pi_cache_equals_gathers():
+    """CP-APR with the other modes' row products kept per nonzero across a
+    mode's inner iterations (the default) equals the run that gathers them
+    in every inner iteration (3- and 4-mode tensors, 64- and 128-bit keys,
+    bench mode and early stops)."""
+    import alto_amd as A
+    for dims, nnz, W, bench in (([40, 300, 500], 40000, 1, False), ([30, 20, 10, 40], 15000, 2, True)):
+        cidx, vals = uniform_tensor(dims, nnz, 21)
+        vals = np.ceil(vals * 4)
+        g = A.AltoTensor(dims, cidx, vals, nprtn=8, lit_bits=64 * W)
+        out = []
+        for cache in (1, 0):
+            A.check(A.lib().alto_tune_set(19, cache), "tune_set")
+            try:
+                U, lam = oracle.random_init(dims, 6, 3)
+                r = g.cp_apr(U, lam, 5, 4, epsilon=1e-4, dobench=bench)
+            finally:
+                A.check(A.lib().alto_tune_set(19, 1), "tune_set")
+            out.append((r, U, lam))
+        (r1, U1, l1), (r0, U0, l0) = out
+        assert r1["outer_iters"] == r0["outer_iters"] and r1["total_inner"] == r0["total_inner"]
+        assert abs(r1["loglik"] - r0["loglik"]) <= 1e-10 * abs(r0["loglik"])
+        for a, b in zip(U1, U0):
+            np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)

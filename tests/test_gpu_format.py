@@ -111,12 +111,14 @@ def test_mttkrp_fp32_bf16_error(R):
 
 
 @pytest.mark.parametrize("case", CASES[:6], ids=lambda c: f"{len(c['dims'])}m_{c['nnz']}")
-@pytest.mark.parametrize("R", [7, 10, 16, 32, 100])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("R", [2, 7, 10, 16, 32, 64, 100])
+@pytest.mark.parametrize("variant", [0, 1, 2])
 def test_phi_fp64(case, R, variant):
-    """Both Phi kernels (runtime batch loop / compile-time batch with DPP
-    broadcasts) match the oracle for 2- to 5-mode tensors and 64- / 128-bit
-    keys."""
+    """The Phi kernels (runtime batch loop / compile-time batch with DPP
+    broadcasts / 16-byte lanes with the model row reloaded only where the row
+    changes; odd ranks take the second) match the oracle for 2- to 5-mode
+    tensors and 64- / 128-bit keys (mode 0 of the second case has runs of
+    about 190 nonzeros per row)."""
     import alto_amd as A
     cidx, vals, g, o = _pair(case)
     rng = np.random.default_rng(R + 1)
