@@ -175,6 +175,8 @@ def main():
                     help="profiling driver of the traffic measurement: the calibration copy, then 3 all-mode MTTKRPs")
     ap.add_argument("--batched-only", action="store_true",
                     help="profiling driver: only the 16-job batched MTTKRP comparison, its JSON on stdout")
+    ap.add_argument("--apr-only", action="store_true",
+                    help="profiling driver: only the CP-APR workload, its JSON on stdout")
     ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the Poisson(50/s) serving stream to run (0 = skip; the full stream is 2000)")
     ap.add_argument("--tune", action="append", default=[], metavar="PARAM=VALUE",
@@ -209,6 +211,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps(batched_leg()), flush=True)
+        return
+    if args.apr_only:
+        print(json.dumps(apr_leg()), flush=True)
         return
 
     t0 = time.time()

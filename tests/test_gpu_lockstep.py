@@ -93,3 +93,66 @@ def test_lockstep_arguments():
     with pytest.raises(A.AltoError):  # one session twice
         A.cpd_sessions_iterate([s0, s0], 2)
     assert A.cpd_sessions_iterate([s0], 0) == [0]
+
+
+def _compare(ts, alone, lock, tol=1e-10):
+    for (dims, _), a, b in zip(ts, alone, lock):
+        Ua, la, fa, _ = a.get()
+        Ub, lb, fb, _ = b.get()
+        assert abs(fa - fb) <= tol, (dims, fa, fb)
+        np.testing.assert_allclose(lb, la, rtol=1e-8)
+        for ua, ub in zip(Ua, Ub):
+            np.testing.assert_allclose(ub, ua, rtol=1e-7, atol=1e-9)
+
+
+def test_lockstep_single_session():
+    """One session in lockstep (solves on the caller's stream, no helper
+    streams) for several iterations equals its own iterate."""
+    A = _A()
+    ts = _tensors(SPECS[1:2], seed0=30)
+    dims, g = ts[0]
+    U, _ = A.kruskal_random_init(dims, 16, 3)
+    alone, lock = [A.CpdSession(g, U)], [A.CpdSession(g, U)]
+    alone[0].iterate(6)
+    assert A.cpd_sessions_iterate(lock, 6) == [6]
+    _compare(ts, alone, lock)
+
+
+def test_lockstep_sessions_on_their_own_streams():
+    """Sessions created on streams of their own, iterated in lockstep on a
+    different stream, equal each session alone on its stream."""
+    import torch
+    A = _A()
+    ts = _tensors(SPECS, seed0=40)
+    streams = [torch.cuda.Stream() for _ in ts]
+    lock_stream = torch.cuda.Stream()
+    alone, lock = [], []
+    for k, ((dims, g), st) in enumerate(zip(ts, streams)):
+        U, _ = A.kruskal_random_init(dims, 16, 20 + k)
+        alone.append(A.CpdSession(g, U, stream=st.cuda_stream))
+        lock.append(A.CpdSession(g, U, stream=st.cuda_stream))
+    for s in alone:
+        s.iterate(5)
+    assert A.cpd_sessions_iterate(lock, 5, stream=lock_stream.cuda_stream) == [5] * len(ts)
+    torch.cuda.synchronize()
+    _compare(ts, alone, lock)
+
+
+def test_lockstep_with_hot_rows():
+    """Rank-ordered streams with LDS-staged hot rows (ALTO_TUNE_HOT_ROWS) in
+    lockstep equal each session alone."""
+    A = _A()
+    A.check(A.lib().alto_tune_set(7, 1), "tune_set")
+    try:
+        ts = _tensors(SPECS[:3] + SPECS[4:], seed0=50)
+        alone, lock = [], []
+        for k, (dims, g) in enumerate(ts):
+            U, _ = A.kruskal_random_init(dims, 16, 30 + k)
+            alone.append(A.CpdSession(g, U))
+            lock.append(A.CpdSession(g, U))
+        for s in alone:
+            s.iterate(5)
+        assert A.cpd_sessions_iterate(lock, 5) == [5] * len(ts)
+        _compare(ts, alone, lock)
+    finally:
+        A.check(A.lib().alto_tune_set(7, 0), "tune_set")
