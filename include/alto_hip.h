@@ -172,7 +172,7 @@ int alto_coo_mttkrp(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64
  * (row, row of a second "fiber" mode -- the other mode with the fewest
  * rows) inside chunks of 2^ALTO_TUNE_FIBER_CHUNK_LOG2 nonzeros (default 30)
  * and split: the nonzeros of fibers of at least ALTO_TUNE_FIBER_MIN
- * (default 4) nonzeros first, walked by a kernel that reads the fiber-mode
+ * (default 3) nonzeros first, walked by a kernel that reads the fiber-mode
  * factor row once per fiber, then the rest.  ALTO_TUNE_FIBER_MODE_RULE: 0 =
  * the fiber mode is the other mode with the fewest rows, 1 = with the most.
  * ALTO_TUNE_DETERMINISTIC: 1 = MTTKRP (every precision, batched calls

@@ -29,7 +29,7 @@ def fibers():
             A.check(A.lib().alto_tune_set(p, v), "tune_set")
 
     yield set_
-    for p, v in ((FIBERS, 1), (FIBER_MIN, 4), (FIBER_CHUNK, 30), (FIBER_MIN_NNZ, 4 << 20), (17, 0)):
+    for p, v in ((FIBERS, 1), (FIBER_MIN, 3), (FIBER_CHUNK, 30), (FIBER_MIN_NNZ, 4 << 20), (17, 0)):
         A.check(A.lib().alto_tune_set(p, v), "tune_set")
 
 

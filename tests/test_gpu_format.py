@@ -156,7 +156,7 @@ def test_mttkrp_accumulate_and_zipf_heavy_rows():
 def tune_reset():
     A = _alto()
     yield A
-    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 32768), (10, 32), (11, 1), (12, 1), (13, 1), (14, 4), (15, 30), (16, 4 << 20), (17, 0)]:
+    for param, value in [(4, -1), (3, 0), (2, 0), (1, 0), (7, 0), (8, 512), (9, 32768), (10, 32), (11, 1), (12, 1), (13, 1), (14, 3), (15, 30), (16, 4 << 20), (17, 0)]:
         A.check(A.lib().alto_tune_set(param, value), "tune_set")
 
 
