@@ -100,11 +100,16 @@ struct ModeStream {
     // this mode, row of mode fib_mode) and split in two parts that each keep
     // that order: first the nonzeros of fibers -- runs of equal (chunk, row,
     // fiber-mode row) -- of at least fib_min nonzeros (nlong of them), then
-    // the others.  MTTKRP reads the fiber-mode row once per fiber in the
-    // first part.
+    // the others.  With a second fiber mode (fib_mode2 >= 0) the others are
+    // ordered and split again in the same way by (chunk, row, row of
+    // fib_mode2): its fibers (nlong2 nonzeros) follow the first part, the
+    // rest comes last.  MTTKRP reads the fiber mode's row once per fiber in
+    // the fiber parts.
     int fib_mode = -1;
+    int fib_mode2 = -1;
     int fib_min = 0;
     uint64_t nlong = 0;
+    uint64_t nlong2 = 0;
 };
 
 }  // namespace alto
@@ -180,9 +185,9 @@ int hot_rows_enabled();
 int reserve_rank_copies(alto_tensor* t, uint64_t rank, size_t elem_bytes, hipStream_t s);
 int split_stream(alto_tensor* t, int mode, const int* hot_k, hipStream_t s);
 int mttkrp_prepare(alto_tensor* t, int R, int precision, hipStream_t s);
-// fiber layout settings (mttkrp.hip): 0 when off; else the minimum fiber
-// length of the first part, the chunk size and the smallest tensor that
-// gets the layout
+// fiber layout settings (mttkrp.hip): 0 when off, else the number of fiber
+// modes (1 or 2); the minimum fiber length of the fiber parts, the chunk
+// size and the smallest tensor that gets the layout
 int fiber_settings(int* min_len, int* chunk_log2, uint64_t* min_nnz);
 int fiber_rule();
 // deterministic-order reductions (ALTO_TUNE_DETERMINISTIC): streams fully

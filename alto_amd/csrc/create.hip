@@ -758,23 +758,27 @@ __global__ void __launch_bounds__(256) k_mode_gather(const uint64_t* __restrict_
 }
 
 // ------------------------------------------------------------------ fiber layout
-// Sort key of a fiber-ordered stream: (chunk, row of `mode`, row of `amode`).
+// Sort key of a fiber-ordered stream: (chunk, row of `mode`, row of `amode`)
+// of the nonzero at ALTO position idx[j] (j itself when idx is null); the
+// chunk is that of the ALTO position.
 template <int W, int PW>
 __global__ void __launch_bounds__(256) k_fiber_key(const uint64_t* __restrict__ keys, LutRef g_lut, PackLayout L,
                                                    int mode, int amode, int chunk_log2, uint64_t n,
-                                                   uint64_t* __restrict__ ck, uint32_t* __restrict__ pay)
+                                                   const uint32_t* __restrict__ idx, uint64_t* __restrict__ ck,
+                                                   uint32_t* __restrict__ pay)
 {
     extern __shared__ uint64_t s_lut[];
     const uint64_t* lut = stage_lut<PW>(s_lut, g_lut);
     const int ba = L.bits[amode], bn = L.bits[mode];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i = idx ? idx[j] : j;
         uint64_t k[W], pk[PW];
 #pragma unroll
         for (int w = 0; w < W; ++w) k[w] = keys[i * W + w];
         lut_apply<W, PW>(lut, g_lut.nb, k, pk);
         const uint64_t r = field<PW>(pk, L.off[mode], bn), a = field<PW>(pk, L.off[amode], ba);
-        ck[i] = (((i >> chunk_log2) << bn | r) << ba) | a;
-        pay[i] = (uint32_t)i;
+        ck[j] = (((i >> chunk_log2) << bn | r) << ba) | a;
+        pay[j] = (uint32_t)i;
     }
 }
 
@@ -1093,68 +1097,104 @@ static bool rank_applicable(const alto_tensor* t)
 
 // the order a mode's stream should have with the current settings
 struct StreamLayout {
-    int chunk_log2, fib_mode, fib_min;
+    int chunk_log2, fib_mode, fib_mode2, fib_min;
 };
 static StreamLayout wanted_layout(const alto_tensor* t, int m, int ranked)
 {
-    StreamLayout L{t->chunk_log2, -1, 0};
+    StreamLayout L{t->chunk_log2, -1, -1, 0};
     if (det_enabled()) {  // one chunk: each row's nonzeros contiguous
         L.chunk_log2 = 63;
         return L;
     }
     int fmin = 0, fchunk = 0;
     uint64_t fnnz = 0;
-    if (ranked || !fiber_settings(&fmin, &fchunk, &fnnz) || t->nmode < 3 || t->nnz < fnnz) return L;
-    int a = -1;  // the fiber mode: the other mode with the fewest (rule 1: most) rows
-    const int most = fiber_rule();
-    for (int q = 0; q < t->nmode; ++q)
-        if (q != m && (a < 0 || (most ? t->dims[q] > t->dims[a] : t->dims[q] < t->dims[a]))) a = q;
+    const int tiers = fiber_settings(&fmin, &fchunk, &fnnz);
+    if (ranked || !tiers || t->nmode < 3 || t->nnz < fnnz) return L;
     const uint64_t nch = (t->nnz + (1ull << fchunk) - 1) >> fchunk;
     int cb = 0;
     while ((1ull << cb) < nch) ++cb;
-    if (cb + t->pack.bits[m] + t->pack.bits[a] > 64) return L;
+    // the fiber modes: the other modes with the fewest (rule 1: most) rows,
+    // in that order, whose sort keys fit 64 bits
+    const int most = fiber_rule();
+    int pick[2] = {-1, -1};
+    for (int k = 0; k < tiers; ++k) {
+        int a = -1;
+        for (int q = 0; q < t->nmode; ++q) {
+            if (q == m || q == pick[0] || cb + t->pack.bits[m] + t->pack.bits[q] > 64) continue;
+            if (a < 0 || (most ? t->dims[q] > t->dims[a] : t->dims[q] < t->dims[a])) a = q;
+        }
+        pick[k] = a;
+        if (a < 0) break;
+    }
+    if (pick[0] < 0) return L;
     L.chunk_log2 = fchunk;
-    L.fib_mode = a;
+    L.fib_mode = pick[0];
+    L.fib_mode2 = pick[1];
     L.fib_min = fmin;
     return L;
 }
 
+// One fiber split: the nonzeros at ALTO positions idx[0, n) (0..n-1 when
+// idx is null) ordered by (chunk, row_m, row_a) and split stably into the
+// nonzeros of fibers of at least `min_len` and the others; the ALTO
+// positions in that order go to perm[0, n), the length of the first part
+// to *nl_dev.
+static int fiber_split(alto_tensor* t, int m, int amode, int chunk_log2, int cb, uint32_t min_len,
+                       const uint32_t* idx, uint64_t n, uint64_t* ck, uint32_t* pay, uint32_t* id, uint32_t* fstart,
+                       uint32_t* cls, uint32_t* perm, DevBuf& tmp, uint64_t* nl_dev, hipStream_t s)
+{
+    DISPATCH_WPW(t->W, t->pack.pw, {
+        hipLaunchKernelGGL((k_fiber_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
+                           t->key.as<uint64_t>(), lutref(t->delin), t->pack, m, amode, chunk_log2, n, idx, ck, pay);
+    });
+    HIP_TRY(hipGetLastError());
+    ALTO_TRY(radix_sort_pairs(ck, 1, pay, n, cb + t->pack.bits[m] + t->pack.bits[amode], s));
+    hipLaunchKernelGGL(k_fiber_heads, dim3(grid_for(n)), dim3(256), 0, s, ck, n, id);
+    ALTO_TRY(scan_exclusive(id, n, tmp, s));
+    hipLaunchKernelGGL(k_fiber_starts, dim3(grid_for(n)), dim3(256), 0, s, ck, n, id, fstart);
+    hipLaunchKernelGGL(k_fiber_class, dim3(grid_for(n)), dim3(256), 0, s, id, fstart, n, min_len, cls);
+    ALTO_TRY(scan_exclusive(cls, n, tmp, s));
+    hipLaunchKernelGGL(k_fiber_scatter, dim3(grid_for(n)), dim3(256), 0, s, id, fstart, cls, pay, n, min_len, perm,
+                       nl_dev);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
 // Orders the nonzeros of stream m by (chunk, row_m, row_a) and splits them
-// into long- and short-fiber parts; the permutation ends in `pay`, the
-// length of the first part in *nlong.  Synchronous on `s`.
+// into long- and short-fiber parts; with a second fiber mode b the short
+// part is ordered by (chunk, row_m, row_b) and split again.  The
+// permutation ends in `pay`, the lengths of the fiber parts in *nlong and
+// *nlong2.  Synchronous on `s`.
 static int fiber_order(alto_tensor* t, int m, const StreamLayout& L, DevBuf& ck, DevBuf& pay, hipStream_t s,
-                       uint64_t* nlong)
+                       uint64_t* nlong, uint64_t* nlong2)
 {
     const uint64_t n = t->nnz;
     const uint64_t nch = (n + (1ull << L.chunk_log2) - 1) >> L.chunk_log2;
     int cb = 0;
     while ((1ull << cb) < nch) ++cb;
-    DISPATCH_WPW(t->W, t->pack.pw, {
-        hipLaunchKernelGGL((k_fiber_key<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,
-                           t->key.as<uint64_t>(), lutref(t->delin), t->pack, m, L.fib_mode, L.chunk_log2, n,
-                           ck.as<uint64_t>(), pay.as<uint32_t>());
-    });
-    HIP_TRY(hipGetLastError());
-    ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n,
-                              cb + t->pack.bits[m] + t->pack.bits[L.fib_mode], s));
     DevBuf id, fstart, cls, perm, tmp, nl;
     ALTO_TRY(id.alloc(n * 4));
     ALTO_TRY(fstart.alloc((n + 1) * 4));
     ALTO_TRY(cls.alloc(n * 4));
     ALTO_TRY(perm.alloc(n * 4));
-    ALTO_TRY(nl.alloc(8));
-    hipLaunchKernelGGL(k_fiber_heads, dim3(grid_for(n)), dim3(256), 0, s, ck.as<uint64_t>(), n, id.as<uint32_t>());
-    ALTO_TRY(scan_exclusive(id.as<uint32_t>(), n, tmp, s));
-    hipLaunchKernelGGL(k_fiber_starts, dim3(grid_for(n)), dim3(256), 0, s, ck.as<uint64_t>(), n, id.as<uint32_t>(),
-                       fstart.as<uint32_t>());
-    hipLaunchKernelGGL(k_fiber_class, dim3(grid_for(n)), dim3(256), 0, s, id.as<uint32_t>(), fstart.as<uint32_t>(), n,
-                       (uint32_t)L.fib_min, cls.as<uint32_t>());
-    ALTO_TRY(scan_exclusive(cls.as<uint32_t>(), n, tmp, s));
-    hipLaunchKernelGGL(k_fiber_scatter, dim3(grid_for(n)), dim3(256), 0, s, id.as<uint32_t>(), fstart.as<uint32_t>(),
-                       cls.as<uint32_t>(), pay.as<uint32_t>(), n, (uint32_t)L.fib_min, perm.as<uint32_t>(),
-                       nl.as<uint64_t>());
-    HIP_TRY(hipGetLastError());
+    ALTO_TRY(nl.alloc(16));
+    uint64_t* ckp = ck.as<uint64_t>();
+    uint32_t *payp = pay.as<uint32_t>(), *permp = perm.as<uint32_t>();
+    ALTO_TRY(fiber_split(t, m, L.fib_mode, L.chunk_log2, cb, (uint32_t)L.fib_min, nullptr, n, ckp, payp,
+                         id.as<uint32_t>(), fstart.as<uint32_t>(), cls.as<uint32_t>(), permp, tmp, nl.as<uint64_t>(),
+                         s));
     HIP_TRY(hipMemcpyAsync(nlong, nl.p, 8, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *nlong2 = 0;
+    const uint64_t rest = n - *nlong;
+    if (L.fib_mode2 >= 0 && rest > 0) {
+        // the short part [nlong, n) of perm holds the ALTO positions it orders
+        const uint64_t o = *nlong;
+        ALTO_TRY(fiber_split(t, m, L.fib_mode2, L.chunk_log2, cb, (uint32_t)L.fib_min, permp + o, rest, ckp + o,
+                             payp + o, id.as<uint32_t>(), fstart.as<uint32_t>(), cls.as<uint32_t>(), permp + o, tmp,
+                             nl.as<uint64_t>() + 1, s));
+        HIP_TRY(hipMemcpyAsync(nlong2, nl.as<uint64_t>() + 1, 8, hipMemcpyDeviceToHost, s));
+    }
     HIP_TRY(hipMemcpyAsync(pay.p, perm.p, n * 4, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
     return ALTO_OK;
@@ -1172,7 +1212,7 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         const ModeStream& M = t->ms[m];
         want[m] = wanted_layout(t, m, ranked);
         if (M.chunk_log2 != want[m].chunk_log2 || M.ranked != ranked || M.fib_mode != want[m].fib_mode ||
-            M.fib_min != want[m].fib_min)
+            M.fib_mode2 != want[m].fib_mode2 || M.fib_min != want[m].fib_min)
             reorder = true;
         if ((need64 && !M.val64.p) || (need32 && !M.val32.p) || !M.pk.p) any = true;
     }
@@ -1183,16 +1223,16 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
     for (int m = 0; m < t->nmode; ++m) {
         ModeStream& M = t->ms[m];
         if (M.chunk_log2 != want[m].chunk_log2 || M.ranked != ranked || M.fib_mode != want[m].fib_mode ||
-            M.fib_min != want[m].fib_min) {
+            M.fib_mode2 != want[m].fib_mode2 || M.fib_min != want[m].fib_min) {
             M.pk.release();
             M.val64.release();
             M.val32.release();
             M.ranked = 0;
             M.split = 0;
             M.pat_start.clear();
-            M.fib_mode = -1;
+            M.fib_mode = M.fib_mode2 = -1;
             M.fib_min = 0;
-            M.nlong = 0;
+            M.nlong = M.nlong2 = 0;
         }
     }
     DevBuf ck, pay;
@@ -1212,11 +1252,13 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
         if (w64 && !M.val64.p) ALTO_TRY(M.val64.alloc(n * 8));
         if (w32 && !M.val32.p) ALTO_TRY(M.val32.alloc(n * 4));
         if (want[m].fib_mode >= 0) {
-            uint64_t nlong = 0;
-            ALTO_TRY(fiber_order(t, m, want[m], ck, pay, s, &nlong));
+            uint64_t nlong = 0, nlong2 = 0;
+            ALTO_TRY(fiber_order(t, m, want[m], ck, pay, s, &nlong, &nlong2));
             M.fib_mode = want[m].fib_mode;
+            M.fib_mode2 = want[m].fib_mode2;
             M.fib_min = want[m].fib_min;
             M.nlong = nlong;
+            M.nlong2 = nlong2;
         } else {
             uint64_t nchunks = (n + (1ull << M.chunk_log2) - 1) >> M.chunk_log2;
             int cbits = 0;
@@ -1228,9 +1270,9 @@ int prepare_mode_streams(alto_tensor* t, int need64, int need32, hipStream_t s)
             });
             HIP_TRY(hipGetLastError());
             ALTO_TRY(radix_sort_pairs(ck.as<uint64_t>(), 1, pay.as<uint32_t>(), n, t->pack.bits[m] + cbits, s));
-            M.fib_mode = -1;
+            M.fib_mode = M.fib_mode2 = -1;
             M.fib_min = 0;
-            M.nlong = 0;
+            M.nlong = M.nlong2 = 0;
         }
         DISPATCH_WPW(t->W, t->pack.pw, {
             hipLaunchKernelGGL((k_mode_gather<W, PW>), dim3(grid_for(n)), dim3(256), lut_smem(t->delin), s,

@@ -63,7 +63,11 @@ static int g_narrow_small = 1;
 static int g_phi_variant = 1;
 // Fiber layout of the mode streams (tensors from 4M nonzeros, 3 or more
 // modes): fibers of at least g_fiber_min nonzeros are walked with the
-// fiber-mode factor row held in registers.
+// fiber-mode factor row held in registers.  g_fibers = number of fiber
+// modes per stream (0 = off; 2: the nonzeros outside the first mode's
+// fibers are searched for fibers of a second mode -- measured slower on the
+// 76.9M-nonzero rank-32 tensor: all-mode FP64 4.48 -> 4.53 ms, FP32 3.27 ->
+// 3.65 ms, although it moves ~10% of the nonzeros into fibers)
 static int g_fibers = 1;
 // 3 (FP32 all-mode MTTKRP of the 76.9M-nonzero rank-32 tensor 3.27 ms
 // against 3.44 at 4; FP64 equal within noise)
@@ -686,7 +690,8 @@ int alto_tune_set(int param, int value)
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_FIBERS) {
-        g_fibers = value ? 1 : 0;
+        if (value < 0 || value > 2) return ALTO_ERR_ARG;
+        g_fibers = value;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_DETERMINISTIC) {

@@ -657,18 +657,18 @@ inline bool rows_aligned(const StreamArgs<FT, VT, OT>& a, int R, int bytes)
     return true;
 }
 
-// The long-fiber part [0, nlong) of a fiber-ordered stream through
-// k_mttkrp_fib (G lanes per nonzero, E columns per lane); `base` carries the
-// stream, value, output and target-mode fields.
+// The fiber parts of a fiber-ordered stream through k_mttkrp_fib (G lanes
+// per nonzero, E columns per lane): [0, nlong) with fiber mode fib_mode,
+// then [nlong, nlong + nlong2) with fib_mode2; `base` carries the stream,
+// value, output and target-mode fields.
 template <typename FT, typename VT, typename AT, typename OT, int E>
-int launch_fiber_part(const alto_tensor* t, int mode, int R, const FT* const* U, const StreamArgs<FT, VT, OT>& base,
-                      int G, int seg, hipStream_t s)
+int launch_fiber_tier(const alto_tensor* t, int mode, int fmode, uint64_t first, uint64_t count, int R,
+                      const FT* const* U, const StreamArgs<FT, VT, OT>& base, int G, int seg, hipStream_t s)
 {
-    const ModeStream& M = t->ms[mode];
     StreamArgs<FT, VT, OT> f = base;
     int o = 0;
     for (int m = 0; m < t->nmode; ++m) {
-        if (m == mode || m == M.fib_mode) continue;
+        if (m == mode || m == fmode) continue;
         f.Uo[o] = U[m];
         f.offo[o] = t->pack.off[m];
         f.bitso[o] = t->pack.bits[m];
@@ -679,11 +679,13 @@ int launch_fiber_part(const alto_tensor* t, int mode, int R, const FT* const* U,
         f.offo[q] = f.bitso[q] = 0;
     }
     f.nother = o;
-    f.nnz = M.nlong;
+    f.pk = base.pk + first * t->pack.pw;
+    f.val = base.val + first;
+    f.nnz = count;
     f.seg = (uint64_t)seg;
     const dim3 grid = stream_grid(f.nnz, seg, G, f);
-    const FT* Ua = U[M.fib_mode];
-    const int offa = t->pack.off[M.fib_mode], bitsa = t->pack.bits[M.fib_mode];
+    const FT* Ua = U[fmode];
+    const int offa = t->pack.off[fmode], bitsa = t->pack.bits[fmode];
 #define KF_(GG, PWW, NBB) \
     hipLaunchKernelGGL((k_mttkrp_fib<FT, VT, AT, OT, GG, E, PWW, NBB>), grid, dim3(256), 0, s, f, Ua, offa, bitsa)
 #define KFN_(GG, PWW)                          \
@@ -713,6 +715,19 @@ int launch_fiber_part(const alto_tensor* t, int mode, int R, const FT* const* U,
     return ALTO_OK;
 }
 
+template <typename FT, typename VT, typename AT, typename OT, int E>
+int launch_fiber_part(const alto_tensor* t, int mode, int R, const FT* const* U, const StreamArgs<FT, VT, OT>& base,
+                      int G, int seg, hipStream_t s)
+{
+    const ModeStream& M = t->ms[mode];
+    if (M.nlong)
+        ALTO_TRY((launch_fiber_tier<FT, VT, AT, OT, E>(t, mode, M.fib_mode, 0, M.nlong, R, U, base, G, seg, s)));
+    if (M.nlong2)
+        ALTO_TRY((launch_fiber_tier<FT, VT, AT, OT, E>(t, mode, M.fib_mode2, M.nlong, M.nlong2, R, U, base, G, seg,
+                                                       s)));
+    return ALTO_OK;
+}
+
 // launch_fiber_part instantiated per element type and columns per lane in
 // mttkrp_fib_{f64,f32,bf16}.hip (translation units of their own, built in
 // parallel with the others)
@@ -723,7 +738,7 @@ int fiber_part(const alto_tensor* t, int mode, int R, const float* const* U, con
 int fiber_part(const alto_tensor* t, int mode, int R, const bf16_t* const* U,
                const StreamArgs<bf16_t, float, double>& a, int G, int E, int seg, hipStream_t s);
 
-// the stream arguments narrowed to the short-fiber part [nlong, nnz)
+// the stream arguments narrowed to the part after the fiber parts
 template <typename FT, typename VT, typename OT>
 inline void skip_long_part(const alto_tensor* t, uint64_t nlong, StreamArgs<FT, VT, OT>& a)
 {
@@ -766,7 +781,8 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
     // FP64 there.
     constexpr bool NARROW_OK = !std::is_same<FT, double>::value;
     const bool narrow_rank = R <= k.narrow_max || (k.narrow_small && t->nnz < (4ull << 20) && R <= 128);
-    const bool fib = M.fib_mode >= 0 && M.nlong > 0 && !hot && a.nother >= 2;
+    const uint64_t nfib = M.nlong + M.nlong2;
+    const bool fib = M.fib_mode >= 0 && nfib > 0 && !hot && a.nother >= 2;
     if (v == 2 && NARROW_OK && narrow_rank && !split && a.nother >= 1 && rows_aligned(a, R, 2 * (int)sizeof(FT))) {
         int G = 1;
         while (G * 2 < R) G <<= 1;
@@ -774,7 +790,7 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
         a.seg = (uint64_t)seg;
         if (fib) {
             ALTO_TRY(fiber_part(t, mode, R, U, a, G, 2, seg, s));
-            skip_long_part(t, M.nlong, a);
+            skip_long_part(t, nfib, a);
             if (a.nnz == 0) return ALTO_OK;
         }
         dim3 grid = stream_grid(a.nnz, seg, G, a);
@@ -813,7 +829,7 @@ int launch_mttkrp(const alto_tensor* t, int mode, int R, const FT* const* U, con
         a.seg = (uint64_t)seg;
         if (fib && !split) {
             ALTO_TRY(fiber_part(t, mode, R, U, a, G, E16, seg, s));
-            skip_long_part(t, M.nlong, a);
+            skip_long_part(t, nfib, a);
             if (a.nnz == 0) return ALTO_OK;
         }
         // one launch per hot-pattern group (one launch over the whole stream

@@ -173,8 +173,11 @@ int alto_coo_mttkrp(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64
  * rows) inside chunks of 2^ALTO_TUNE_FIBER_CHUNK_LOG2 nonzeros (default 30)
  * and split: the nonzeros of fibers of at least ALTO_TUNE_FIBER_MIN
  * (default 3) nonzeros first, walked by a kernel that reads the fiber-mode
- * factor row once per fiber, then the rest.  ALTO_TUNE_FIBER_MODE_RULE: 0 =
- * the fiber mode is the other mode with the fewest rows, 1 = with the most.
+ * factor row once per fiber, then the rest.  2 = the rest is ordered and
+ * split again by (row, row of the other mode with the next fewest rows):
+ * its fibers follow the first part.  0 = off.
+ * ALTO_TUNE_FIBER_MODE_RULE: 0 = the fiber modes are the other modes with
+ * the fewest rows, 1 = with the most.
  * ALTO_TUNE_DETERMINISTIC: 1 = MTTKRP (every precision, batched calls
  * included) and Phi sum in one fixed order, with no floating-point atomics:
  * the mode streams are fully sorted by row, each segment of 1024 nonzeros
