@@ -281,6 +281,10 @@ struct CpdSessionBase {
     virtual void mode_io(int j, const void** fptr, void** M) = 0;
     virtual int mode_rest(int j, hipStream_t st) = 0;
     virtual int fit_launch(hipStream_t st, double* fit_dst) = 0;
+    // the same pieces as jobs of the batched dense launches (rank <= 64):
+    // the rest of mode j, and the fit (its value also copied to *gather)
+    virtual void mode_job(int j, DenseModeJob* x) = 0;
+    virtual void fit_job(double* gather, DenseFitJob* f) = 0;
 };
 
 int graphs_enabled();
@@ -377,6 +381,44 @@ struct CpdSession : CpdSessionBase {
         ALTO_TRY(cpd_fit_step(t, c, normX, st));
         HIP_TRY(hipMemcpyAsync(fit_dst, c.fitbuf.p, sizeof(double), hipMemcpyDeviceToHost, st));
         return fence(st);
+    }
+    // the arguments cpd_mode_rest and cpd_fit_step pass to their launches
+    void mode_job(int j, DenseModeJob* x) override
+    {
+        static_assert(std::is_same<MT, double>::value, "the row solves read an FP64 M");
+        const int N = c.N;
+        x->grams = c.grams.template as<double>();
+        x->nmode = N;
+        x->mode = j;
+        x->P = c.P.template as<double>();
+        x->wsA = c.wsA.template as<double>();
+        x->wsQ = c.wsQ.template as<double>();
+        x->status = c.status.template as<int>();
+        x->M = c.M.template as<double>();
+        x->I = t->dims[j];
+        x->U = c.u(j);
+        x->psum = c.psum.template as<double>();
+        x->pmax = c.pmax.template as<double>();
+        x->pin = j == N - 1 ? c.pin.template as<double>() : nullptr;
+        x->nb = c.nb_solve;
+        x->which = it == 0 ? 0 : 1;
+        x->lambda = c.lambda.template as<double>();
+        x->S = std::is_same<UT, ST>::value ? nullptr : (void*)c.st(j);
+        x->G = c.grams.template as<double>() + (size_t)j * R * R;
+        x->gpart = c.gpart.template as<double>();
+        x->zero = c.M.p;
+        x->zero_bytes = t->dims[(j + 1) % N] * R * sizeof(MT);
+    }
+    void fit_job(double* gather, DenseFitJob* f) override
+    {
+        f->grams = c.grams.template as<double>();
+        f->lambda = c.lambda.template as<double>();
+        f->pin = c.pin.template as<double>();
+        f->out = c.fitbuf.template as<double>();
+        f->gather = gather;
+        f->normX = normX;
+        f->nmode = c.N;
+        f->nb = c.nb_solve;
     }
     int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) override
     {
@@ -688,6 +730,23 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
         return ALTO_OK;
     };
     auto lane = [&](int q) { return naux ? aux.st[q % naux] : s; };
+    // rank <= 64: each mode's updates and the fits of all sessions in one
+    // launch per kernel (in groups of DENSE_BATCH_MAX), on `s`
+    const int R = (int)b[0]->R, prec = b[0]->prec;
+    const bool dense_batched = lockstep_dense_batch_enabled() && R <= 64;
+    DevBuf gathered;
+    if (dense_batched) ALTO_TRY(gathered.alloc((size_t)n * sizeof(double)));
+    std::vector<DenseModeJob> djobs;
+    std::vector<DenseFitJob> fjobs;
+    auto dense_groups = [&]() -> int {
+        for (size_t k0 = 0; k0 < djobs.size(); k0 += DENSE_BATCH_MAX) {
+            const int cnt = (int)std::min<size_t>(DENSE_BATCH_MAX, djobs.size() - k0);
+            if (prec == ALTO_PREC_F64) ALTO_TRY((dense_mode_batch<double, double>(cnt, &djobs[k0], R, true, s)));
+            else if (prec == ALTO_PREC_F32) ALTO_TRY((dense_mode_batch<float, float>(cnt, &djobs[k0], R, false, s)));
+            else ALTO_TRY((dense_mode_batch<float, bf16_t>(cnt, &djobs[k0], R, false, s)));
+        }
+        return ALTO_OK;
+    };
     std::vector<alto_tensor_t*> ts;
     std::vector<int> modes;
     std::vector<std::vector<const void*>> fp(n, std::vector<const void*>(ALTO_MAXN));
@@ -721,16 +780,38 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
             // M is zero here (cleared at set-up and by each mode's last launch)
             ALTO_TRY(mttkrp_batched_impl((int)ts.size(), ts.data(), modes.data(), (uint64_t)b[0]->R, fpp.data(),
                                          outs.data(), b[0]->prec, 1, s));
+            if (dense_batched) {
+                djobs.clear();
+                for (int i : act)
+                    if (j < b[i]->nmodes()) {
+                        djobs.emplace_back();
+                        b[i]->mode_job(j, &djobs.back());
+                    }
+                ALTO_TRY(dense_groups());
+                continue;
+            }
             ALTO_TRY(fork());
             int q = 0;
             for (int i : act)
                 if (j < b[i]->nmodes()) ALTO_TRY(b[i]->mode_rest(j, lane(q++)));
             ALTO_TRY(join());
         }
-        ALTO_TRY(fork());
-        int q = 0;
-        for (int i : act) ALTO_TRY(b[i]->fit_launch(lane(q++), fits + i));
-        ALTO_TRY(join());
+        if (dense_batched) {
+            fjobs.clear();
+            for (int i : act) {
+                fjobs.emplace_back();
+                b[i]->fit_job(gathered.as<double>() + i, &fjobs.back());
+            }
+            for (size_t k0 = 0; k0 < fjobs.size(); k0 += DENSE_BATCH_MAX)
+                ALTO_TRY(fit_batch((int)std::min<size_t>(DENSE_BATCH_MAX, fjobs.size() - k0), &fjobs[k0], R, s));
+            HIP_TRY(hipMemcpyAsync(fits, gathered.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
+            for (int i : act) ALTO_TRY(b[i]->fence(s));
+        } else {
+            ALTO_TRY(fork());
+            int q = 0;
+            for (int i : act) ALTO_TRY(b[i]->fit_launch(lane(q++), fits + i));
+            ALTO_TRY(join());
+        }
         HIP_TRY(hipStreamSynchronize(s));
         const double dt = now_s() - t0;
         for (int i : act) {

@@ -99,9 +99,9 @@ __global__ void __launch_bounds__(256) k_gram_partial(const T* __restrict__ U, u
 // the Gram is taken of the stored values — the column normalisation and the
 // Gram update of a mode in one pass over U.
 template <typename T, typename ST, bool F64, int T16, bool SCALE>
-__global__ void __launch_bounds__(256) k_gram_rows(T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
-                                                   double* __restrict__ part, const double* __restrict__ lambda,
-                                                   ST* __restrict__ S)
+__device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
+                                               double* __restrict__ part, const double* __restrict__ lambda,
+                                               ST* __restrict__ S, uint32_t bid)
 {
     constexpr int NP = T16 * (T16 + 1) / 2;
     using acc_t = typename std::conditional<F64, d4_t, f4_t>::type;
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_gram_rows(T* __restrict__ U, uint64_t I
     acc_t acc[NP];
 #pragma unroll
     for (int q = 0; q < NP; ++q) acc[q] = acc_t{0, 0, 0, 0};
-    const uint64_t base = (uint64_t)blockIdx.x * rows_per_block;
+    const uint64_t base = (uint64_t)bid * rows_per_block;
     const uint64_t end = base + rows_per_block < I ? base + rows_per_block : I;
     double lc[T16];
     if constexpr (SCALE) {
@@ -187,12 +187,20 @@ __global__ void __launch_bounds__(256) k_gram_rows(T* __restrict__ U, uint64_t I
         }
         __syncthreads();
     }
-    double* P = part + (size_t)blockIdx.x * R * R;
+    double* P = part + (size_t)bid * R * R;
     for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
         const int a = k / R, b = k % R;
         // the upper block triangle holds the sums; mirror the lower one
         P[k] = (a / 16 <= b / 16) ? sG[k] : sG[b * R + a];
     }
+}
+
+template <typename T, typename ST, bool F64, int T16, bool SCALE>
+__global__ void __launch_bounds__(256) k_gram_rows(T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
+                                                   double* __restrict__ part, const double* __restrict__ lambda,
+                                                   ST* __restrict__ S)
+{
+    gram_rows_body<T, ST, F64, T16, SCALE>(U, I, R, rows_per_block, part, lambda, S, blockIdx.x);
 }
 
 __global__ void k_sum_partials(const double* __restrict__ part, int nparts, int n, double* __restrict__ out)
@@ -209,13 +217,13 @@ __global__ void k_sum_partials(const double* __restrict__ part, int nparts, int 
 // order (deterministic).  Blocks from nsum on clear `zero` (nz bytes, a
 // multiple of 4): a buffer the caller needs zeroed next, for free in this
 // launch.
-__global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ part, int nparts, int n,
-                                                  double* __restrict__ out, int nsum, void* __restrict__ zero,
-                                                  uint64_t nz)
+__device__ __forceinline__ void sum_cols_body(const double* __restrict__ part, int nparts, int n,
+                                              double* __restrict__ out, int nsum, void* __restrict__ zero, uint64_t nz,
+                                              uint32_t bid, uint32_t nblk)
 {
-    if ((int)blockIdx.x >= nsum) {
-        const uint64_t t = (uint64_t)(blockIdx.x - nsum) * blockDim.x + threadIdx.x;
-        const uint64_t stride = (uint64_t)(gridDim.x - nsum) * blockDim.x;
+    if ((int)bid >= nsum) {
+        const uint64_t t = (uint64_t)(bid - nsum) * blockDim.x + threadIdx.x;
+        const uint64_t stride = (uint64_t)(nblk - nsum) * blockDim.x;
         uint4* z16 = reinterpret_cast<uint4*>(zero);
         const uint64_t n16 = nz / 16;
         for (uint64_t i = t; i < n16; i += stride) z16[i] = make_uint4(0, 0, 0, 0);
@@ -225,7 +233,7 @@ __global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ par
     }
     __shared__ double s[8][33];
     const int kk = threadIdx.x & 31, pg = threadIdx.x >> 5;
-    const int k = blockIdx.x * 32 + kk;
+    const int k = bid * 32 + kk;
     double a = 0;
     if (k < n)
         for (int p = pg; p < nparts; p += 8) a += part[(size_t)p * n + k];
@@ -237,6 +245,13 @@ __global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ par
         for (int g = 0; g < 8; ++g) v += s[g][kk];
         out[k] = v;
     }
+}
+
+__global__ void __launch_bounds__(256) k_sum_cols(const double* __restrict__ part, int nparts, int n,
+                                                  double* __restrict__ out, int nsum, void* __restrict__ zero,
+                                                  uint64_t nz)
+{
+    sum_cols_body(part, nparts, n, out, nsum, zero, nz, blockIdx.x, gridDim.x);
 }
 
 // the column sums of n-entry partials (n > 0) and the clearing of `zero`
@@ -414,9 +429,9 @@ __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ g
                                                     double* __restrict__ Pout);
 
 template <int BT, int NPT>
-__global__ void __launch_bounds__(BT) k_spd_inverse(const double* __restrict__ grams, int nmode, int mode, int R,
-                                                    double* __restrict__ W, int* __restrict__ status,
-                                                    double* __restrict__ wsA, double* __restrict__ wsQ)
+__device__ __forceinline__ void spd_inverse_body(const double* __restrict__ grams, int nmode, int mode, int R,
+                                                 double* __restrict__ W, int* __restrict__ status,
+                                                 double* __restrict__ wsA, double* __restrict__ wsQ)
 {
     __shared__ double s_col[2][128], s_row[2][128];
     const int tid = threadIdx.x;
@@ -488,6 +503,14 @@ __global__ void __launch_bounds__(BT) k_spd_inverse(const double* __restrict__ g
     if (tid == 0) *status = 1;
     __syncthreads();
     pinv_body(grams, nmode, mode, R, wsA, wsQ, W);
+}
+
+template <int BT, int NPT>
+__global__ void __launch_bounds__(BT) k_spd_inverse(const double* __restrict__ grams, int nmode, int mode, int R,
+                                                    double* __restrict__ W, int* __restrict__ status,
+                                                    double* __restrict__ wsA, double* __restrict__ wsQ)
+{
+    spd_inverse_body<BT, NPT>(grams, nmode, mode, R, W, status, wsA, wsQ);
 }
 
 // Minimum-norm pseudo-inverse V^+ = Q diag(1/e) Q^T over |e| > R*eps*max|e|
@@ -629,9 +652,9 @@ __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ g
 // The 16 x R block of M is loaded into registers before the products, so
 // its loads are in flight together.  T16: 16-column tiles (R <= 16 T16).
 template <typename MT, typename UT, int T16>
-__global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ Wg,
-                                               UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
-                                               double* __restrict__ pin)
+__device__ __forceinline__ void apply_body(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ Wg,
+                                           UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
+                                           double* __restrict__ pin, uint32_t bid, uint32_t nblk)
 {
     // FP32 / BF16 models (U in FP32; M in FP64 or FP32): the products on
     // FP32 MFMA with M and the FP64 inverse rounded to FP32; FP64 models:
@@ -654,7 +677,7 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
         mx[t] = -INFINITY;
     }
     double in = 0;
-    for (uint64_t row0 = ((uint64_t)blockIdx.x * 4 + wave) * 16; row0 < I; row0 += (uint64_t)gridDim.x * 64) {
+    for (uint64_t row0 = ((uint64_t)bid * 4 + wave) * 16; row0 < I; row0 += (uint64_t)nblk * 64) {
         const uint64_t ra = row0 + r16;
         mma_t av[KS];
 #pragma unroll
@@ -721,8 +744,8 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
             a += s_ss[(w * T16 + t) * 16 + l];
             b = fmax(b, s_mx[(w * T16 + t) * 16 + l]);
         }
-        psum[(size_t)blockIdx.x * R + c] = a;
-        pmax[(size_t)blockIdx.x * R + c] = b;
+        psum[(size_t)bid * R + c] = a;
+        pmax[(size_t)bid * R + c] = b;
     }
     if (pin) {
         __shared__ double s_in[4];
@@ -730,17 +753,24 @@ __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_
         for (int o = 32; o > 0; o >>= 1) in += __shfl_xor(in, o);
         if (lane == 0) s_in[wave] = in;
         __syncthreads();
-        if (threadIdx.x == 0) pin[blockIdx.x] = (s_in[0] + s_in[1]) + (s_in[2] + s_in[3]);
+        if (threadIdx.x == 0) pin[bid] = (s_in[0] + s_in[1]) + (s_in[2] + s_in[3]);
     }
+}
+
+template <typename MT, typename UT, int T16>
+__global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ Wg,
+                                               UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
+                                               double* __restrict__ pin)
+{
+    apply_body<MT, UT, T16>(M, I, R, Wg, U, psum, pmax, pin, blockIdx.x, gridDim.x);
 }
 
 // lambda: iteration 0 -> 2-norm of each column; later -> max(1, max_i U[i,r]).
 // One block per column, tree reduction over the per-block partials.
-__global__ void __launch_bounds__(256) k_lambda(const double* __restrict__ psum, const double* __restrict__ pmax, int nb,
-                                                int R, int which, double* __restrict__ lambda)
+__device__ __forceinline__ void lambda_body(const double* __restrict__ psum, const double* __restrict__ pmax, int nb,
+                                            int R, int which, double* __restrict__ lambda, int c)
 {
     __shared__ double sa[256], sb[256];
-    const int c = blockIdx.x;
     double a = 0, b = -INFINITY;
     for (int p = threadIdx.x; p < nb; p += blockDim.x) {
         a += psum[(size_t)p * R + c];
@@ -757,6 +787,12 @@ __global__ void __launch_bounds__(256) k_lambda(const double* __restrict__ psum,
         __syncthreads();
     }
     if (threadIdx.x == 0) lambda[c] = which == 0 ? sqrt(sa[0]) : fmax(1.0, fmax(sb[0], 0.0));
+}
+
+__global__ void __launch_bounds__(256) k_lambda(const double* __restrict__ psum, const double* __restrict__ pmax, int nb,
+                                                int R, int which, double* __restrict__ lambda)
+{
+    lambda_body(psum, pmax, nb, R, which, lambda, blockIdx.x);
 }
 
 // U /= lambda (per column), with the storage copy used by MTTKRP
@@ -798,8 +834,9 @@ __global__ void k_inner_partial(const UT* __restrict__ U, const MT* __restrict__
 // when res > 0, as in the reference)
 // inner from per-column partials (nb x R, weighted by lambda) or, with
 // scalar set, from nb per-block totals already carrying lambda
-__global__ void k_fit(const double* __restrict__ grams, int nmode, int R, const double* __restrict__ lambda,
-                      const double* __restrict__ ipart, int nb, int scalar, double normX, double* __restrict__ out)
+__device__ __forceinline__ void fit_body(const double* __restrict__ grams, int nmode, int R,
+                                         const double* __restrict__ lambda, const double* __restrict__ ipart, int nb,
+                                         int scalar, double normX, double* __restrict__ out)
 {
     __shared__ double s_in[256], s_nu[256];
     double inner = 0, nu = 0;
@@ -837,6 +874,13 @@ __global__ void k_fit(const double* __restrict__ grams, int nmode, int R, const 
         out[1] = s_in[0];
         out[2] = normU;
     }
+}
+
+__global__ void __launch_bounds__(256) k_fit(const double* __restrict__ grams, int nmode, int R,
+                                             const double* __restrict__ lambda, const double* __restrict__ ipart,
+                                             int nb, int scalar, double normX, double* __restrict__ out)
+{
+    fit_body(grams, nmode, R, lambda, ipart, nb, scalar, normX, out);
 }
 
 __global__ void k_sumsq_partial(const double* __restrict__ v, uint64_t n, double* __restrict__ part)
@@ -987,6 +1031,200 @@ int normsq_device(const double* v, uint64_t n, double* part, double* out, hipStr
     const int nb = 256;
     hipLaunchKernelGGL(k_sumsq_partial, dim3(nb), dim3(256), 0, s, v, n, part);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1), 0, s, part, nb, 1, out);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+
+// ------------------------------------------------------------------ batched mode updates
+// The mode-j update of several CP-ALS models (lockstep sessions of one rank
+// and precision) in one launch per kernel: each kernel's table (up to
+// DENSE_BATCH_MAX jobs, in the kernel arguments) gives job k the logical
+// blocks [start[k], start[k + 1]) -- its own launch's grid -- and each block
+// finds its job by a uniform scan.  The bodies are the single-model
+// kernels', so every job computes what its own launches would.
+template <typename J>
+struct BatchTab {
+    J a[DENSE_BATCH_MAX];
+    uint32_t start[DENSE_BATCH_MAX + 1];
+    int n;
+};
+template <typename J>
+__device__ inline int batch_job(const BatchTab<J>& t, uint32_t b)
+{
+    int k = 0;
+    while (k + 1 < t.n && b >= t.start[k + 1]) ++k;
+    return k;
+}
+
+struct SpdJob {
+    const double* grams;
+    double *P, *wsA, *wsQ;
+    int* status;
+    int nmode, mode;
+};
+struct ApplyJob {
+    const double* M;
+    const double* P;
+    void* U;
+    double *psum, *pmax, *pin;
+    uint64_t I;
+};
+struct LambdaJob {
+    const double *psum, *pmax;
+    double* lambda;
+    int nb, which;
+};
+struct GramJob {
+    void* U;
+    void* S;
+    const double* lambda;
+    double* part;
+    uint64_t I, rpb;
+};
+struct SumJob {
+    const double* part;
+    double* out;
+    void* zero;
+    uint64_t nz;
+    int nparts, nsum;
+};
+struct FitJob {
+    const double *grams, *lambda, *pin;
+    double* out;
+    double* gather;
+    double normX;
+    int nmode, nb;
+};
+
+template <int BT, int NPT>
+__global__ void __launch_bounds__(BT) k_spd_inverse_batch(BatchTab<SpdJob> t, int R)
+{
+    const SpdJob& j = t.a[blockIdx.x];
+    spd_inverse_body<BT, NPT>(j.grams, j.nmode, j.mode, R, j.P, j.status, j.wsA, j.wsQ);
+}
+
+template <typename MT, typename UT, int T16>
+__global__ void __launch_bounds__(256) k_apply_batch(BatchTab<ApplyJob> t, int R)
+{
+    const int k = batch_job(t, blockIdx.x);
+    const ApplyJob& j = t.a[k];
+    apply_body<MT, UT, T16>(j.M, j.I, R, j.P, (UT*)j.U, j.psum, j.pmax, j.pin, blockIdx.x - t.start[k],
+                            t.start[k + 1] - t.start[k]);
+}
+
+__global__ void __launch_bounds__(256) k_lambda_batch(BatchTab<LambdaJob> t, int R)
+{
+    const LambdaJob& j = t.a[blockIdx.x / R];
+    lambda_body(j.psum, j.pmax, j.nb, R, j.which, j.lambda, blockIdx.x % R);
+}
+
+template <typename T, typename ST, bool F64, int T16>
+__global__ void __launch_bounds__(256) k_gram_rows_batch(BatchTab<GramJob> t, int R)
+{
+    const int k = batch_job(t, blockIdx.x);
+    const GramJob& j = t.a[k];
+    gram_rows_body<T, ST, F64, T16, true>((T*)j.U, j.I, R, j.rpb, j.part, j.lambda, (ST*)j.S,
+                                          blockIdx.x - t.start[k]);
+}
+
+__global__ void __launch_bounds__(256) k_sum_cols_batch(BatchTab<SumJob> t, int n)
+{
+    const int k = batch_job(t, blockIdx.x);
+    const SumJob& j = t.a[k];
+    sum_cols_body(j.part, j.nparts, n, j.out, j.nsum, j.zero, j.nz, blockIdx.x - t.start[k],
+                  t.start[k + 1] - t.start[k]);
+}
+
+__global__ void __launch_bounds__(256) k_fit_batch(BatchTab<FitJob> t, int R)
+{
+    const FitJob& j = t.a[blockIdx.x];
+    fit_body(j.grams, j.nmode, R, j.lambda, j.pin, j.nb, 1, j.normX, j.out);
+    if (threadIdx.x == 0) j.gather[0] = j.out[0];
+}
+
+template <typename UT, typename ST>
+int dense_mode_batch(int n, const DenseModeJob* m, int R, bool f64, hipStream_t s)
+{
+    if (n < 1 || n > DENSE_BATCH_MAX || R < 1 || R > 64) {
+        set_error("dense_mode_batch: %d jobs of rank %d (1..%d jobs, rank 1..64)", n, R, DENSE_BATCH_MAX);
+        return ALTO_ERR_ARG;
+    }
+    constexpr bool same = std::is_same<UT, ST>::value;
+    BatchTab<SpdJob> sp{};
+    BatchTab<ApplyJob> ap{};
+    BatchTab<LambdaJob> la{};
+    BatchTab<GramJob> gr{};
+    BatchTab<SumJob> su{};
+    sp.n = ap.n = la.n = gr.n = su.n = n;
+    ap.start[0] = gr.start[0] = su.start[0] = 0;
+    for (int k = 0; k < n; ++k) {
+        const DenseModeJob& x = m[k];
+        sp.a[k] = SpdJob{x.grams, x.P, x.wsA, x.wsQ, x.status, x.nmode, x.mode};
+        ap.a[k] = ApplyJob{x.M, x.P, x.U, x.psum, x.pmax, x.pin, x.I};
+        ap.start[k + 1] = ap.start[k] + (uint32_t)x.nb;
+        la.a[k] = LambdaJob{x.psum, x.pmax, x.lambda, x.nb, x.which};
+        const int gb = gram_blocks(x.I);
+        const uint64_t rpb = ((x.I + gb - 1) / gb + 15) / 16 * 16;
+        gr.a[k] = GramJob{x.U, same ? nullptr : x.S, x.lambda, x.gpart, x.I, rpb};
+        gr.start[k + 1] = gr.start[k] + (uint32_t)gb;
+        // the sums and the clearing of the next mode's M, as sum_cols_device
+        const int nsum = (R * R + 31) / 32;
+        uint64_t nzb = (x.zero_bytes / 16 + 255) / 256;
+        if (nzb > 1024) nzb = 1024;
+        if (x.zero_bytes && nzb == 0) nzb = 1;
+        su.a[k] = SumJob{x.gpart, x.G, x.zero, x.zero_bytes, gb, nsum};
+        su.start[k + 1] = su.start[k] + (uint32_t)(nsum + nzb);
+    }
+#define SPD_(BT, NPT) hipLaunchKernelGGL((k_spd_inverse_batch<BT, NPT>), dim3(n), dim3(BT), 0, s, sp, R)
+    if (R <= 16) SPD_(256, 1);
+    else if (R <= 32) SPD_(1024, 1);
+    else SPD_(1024, 4);
+#undef SPD_
+    const size_t lds_ap = (size_t)R * (R + 2) * 8;
+#define AP_(TT) hipLaunchKernelGGL((k_apply_batch<double, UT, TT>), dim3(ap.start[n]), dim3(256), lds_ap, s, ap, R)
+    switch ((R + 15) / 16) {
+    case 1: AP_(1); break;
+    case 2: AP_(2); break;
+    case 3: AP_(3); break;
+    default: AP_(4); break;
+    }
+#undef AP_
+    hipLaunchKernelGGL(k_lambda_batch, dim3(n * R), dim3(256), 0, s, la, R);
+    const size_t lds_gr = (size_t)R * R * 8;
+#define GS_(TT)                                                                                                     \
+    do {                                                                                                            \
+        if (f64)                                                                                                    \
+            hipLaunchKernelGGL((k_gram_rows_batch<UT, ST, true, TT>), dim3(gr.start[n]), dim3(256), lds_gr, s, gr, R); \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_gram_rows_batch<UT, ST, false, TT>), dim3(gr.start[n]), dim3(256), lds_gr, s, gr, \
+                               R);                                                                                  \
+    } while (0)
+    switch ((R + 15) / 16) {
+    case 1: GS_(1); break;
+    case 2: GS_(2); break;
+    case 3: GS_(3); break;
+    default: GS_(4); break;
+    }
+#undef GS_
+    hipLaunchKernelGGL(k_sum_cols_batch, dim3(su.start[n]), dim3(256), 0, s, su, R * R);
+    HIP_TRY(hipGetLastError());
+    return ALTO_OK;
+}
+template int dense_mode_batch<double, double>(int, const DenseModeJob*, int, bool, hipStream_t);
+template int dense_mode_batch<float, float>(int, const DenseModeJob*, int, bool, hipStream_t);
+template int dense_mode_batch<float, bf16_t>(int, const DenseModeJob*, int, bool, hipStream_t);
+
+int fit_batch(int n, const DenseFitJob* f, int R, hipStream_t s)
+{
+    if (n < 1 || n > DENSE_BATCH_MAX) {
+        set_error("fit_batch: %d jobs (1..%d)", n, DENSE_BATCH_MAX);
+        return ALTO_ERR_ARG;
+    }
+    BatchTab<FitJob> t{};
+    t.n = n;
+    for (int k = 0; k < n; ++k)
+        t.a[k] = FitJob{f[k].grams, f[k].lambda, f[k].pin, f[k].out, f[k].gather, f[k].normX, f[k].nmode, f[k].nb};
+    hipLaunchKernelGGL(k_fit_batch, dim3(n), dim3(256), 0, s, t, R);
     HIP_TRY(hipGetLastError());
     return ALTO_OK;
 }

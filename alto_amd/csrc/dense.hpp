@@ -38,4 +38,43 @@ int normsq_device(const double* v, uint64_t n, double* part, double* out, hipStr
 // alto_update_gram_noalloc (partials in a per-device scratch array)
 int gram_noalloc(const void* U, uint64_t dim, int R, int precision, double* gram, hipStream_t s);
 
+// One CP-ALS model's mode update after its MTTKRP (M = M_mode): V^-1 from
+// the other modes' Grams, U = M V^-1 with the column statistics (and the
+// fit's inner-product partials when pin is set), lambda (which: 0 = 2-norm,
+// 1 = max-norm), U /= lambda with the storage copy S and the new Gram G,
+// and the clearing of zero_bytes of `zero` -- the launches of chol_device,
+// solve_device, lambda_device and gram_scale_device for R <= 64
+struct DenseModeJob {
+    const double* grams;
+    int nmode, mode;
+    double *P, *wsA, *wsQ;
+    int* status;
+    const double* M;
+    uint64_t I;
+    void* U;
+    double *psum, *pmax, *pin;
+    int nb;  // row-solve blocks (the psum / pmax partials)
+    int which;
+    double* lambda;
+    void* S;
+    double *G, *gpart;
+    void* zero;
+    uint64_t zero_bytes;
+};
+constexpr int DENSE_BATCH_MAX = 16;
+// the mode updates of n <= DENSE_BATCH_MAX models of one rank (<= 64) and
+// factor / storage types in 5 launches (one per kernel, all models each)
+template <typename UT, typename ST>
+int dense_mode_batch(int n, const DenseModeJob* jobs, int R, bool f64, hipStream_t s);
+// the fits of n <= DENSE_BATCH_MAX models (fit_scalar_device each), one
+// launch; each fit is also copied to gather[0]
+struct DenseFitJob {
+    const double *grams, *lambda, *pin;
+    double* out;
+    double* gather;
+    double normX;
+    int nmode, nb;
+};
+int fit_batch(int n, const DenseFitJob* jobs, int R, hipStream_t s);
+
 }  // namespace alto

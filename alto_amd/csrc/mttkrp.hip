@@ -91,6 +91,9 @@ int det_enabled() { return g_det; }
 static int g_apr_pi_cache = 1;
 int apr_pi_cache_enabled() { return g_apr_pi_cache; }
 int graphs_enabled() { return g_graphs; }
+// lockstep CP-ALS: the sessions' mode updates and fits in batched launches
+static int g_lockstep_dense = 1;
+int lockstep_dense_batch_enabled() { return g_lockstep_dense; }
 int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
@@ -696,6 +699,10 @@ int alto_tune_set(int param, int value)
     }
     if (param == ALTO_TUNE_DETERMINISTIC) {
         g_det = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_LOCKSTEP_DENSE_BATCH) {
+        g_lockstep_dense = value ? 1 : 0;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_APR_PI_CACHE) {

@@ -761,11 +761,20 @@ def batched_leg(n_jobs=16, reps=50):
             f"one batched call {res['batched_call_ms']:.3f} ms (x{res['speedup']:.2f})")
     del keep
     # CP-ALS on the same tensors: 20 iterations of each session one after
-    # another against all sessions in lockstep (alto_cpd_sessions_iterate)
-    for R, prec, name in ((16, A.PREC_F64, "cpals_r16_fp64"), (16, A.PREC_F32, "cpals_r16_fp32")):
+    # another against all sessions in lockstep (alto_cpd_sessions_iterate),
+    # with the dense steps batched (the default) and as per-session chains
+    for R, prec, name in ((16, A.PREC_F64, "cpals_r16_fp64"), (16, A.PREC_F32, "cpals_r16_fp32"),
+                          (32, A.PREC_F64, "cpals_r32_fp64"), (64, A.PREC_F64, "cpals_r64_fp64")):
         inits = [A.kruskal_random_init(dims, R, 50 + k)[0] for k, (_, dims, _) in enumerate(tensors)]
+        # first launches of this rank's kernels (code loading) outside the timing
+        warm = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
+        for x in warm:
+            x.iterate(1)
+        A.cpd_sessions_iterate(warm, 1)
+        del warm
         alone = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
         lock = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
+        chains = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for x in alone:
@@ -775,12 +784,23 @@ def batched_leg(n_jobs=16, reps=50):
         A.cpd_sessions_iterate(lock, 20)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        dfit = max(abs(x.get()[2] - y.get()[2]) for x, y in zip(alone, lock))
+        A.check(L.alto_tune_set(20, 0), "tune_set")  # ALTO_TUNE_LOCKSTEP_DENSE_BATCH
+        try:
+            t3 = time.perf_counter()
+            A.cpd_sessions_iterate(chains, 20)
+            torch.cuda.synchronize()
+            t4 = time.perf_counter()
+        finally:
+            A.check(L.alto_tune_set(20, 1), "tune_set")
+        dfit = max(max(abs(x.get()[2] - y.get()[2]), abs(x.get()[2] - z.get()[2]))
+                   for x, y, z in zip(alone, lock, chains))
         out[name] = {"iterations": 20, "one_after_another_ms": (t1 - t0) * 1e3, "lockstep_ms": (t2 - t1) * 1e3,
+                     "lockstep_per_session_chains_ms": (t4 - t3) * 1e3,
                      "speedup": (t1 - t0) / (t2 - t1), "max_fit_delta": dfit}
         log(f"lockstep CP-ALS {name}: {n_jobs} sessions x 20 iterations, one after another {(t1 - t0) * 1e3:.1f} ms, "
-            f"lockstep {(t2 - t1) * 1e3:.1f} ms (x{(t1 - t0) / (t2 - t1):.2f}), max fit delta {dfit:.1e}")
-        del alone, lock
+            f"lockstep {(t2 - t1) * 1e3:.1f} ms (x{(t1 - t0) / (t2 - t1):.2f}; with per-session dense chains "
+            f"{(t4 - t3) * 1e3:.1f} ms), max fit delta {dfit:.1e}")
+        del alone, lock, chains
     del tensors
     torch.cuda.empty_cache()
     return out

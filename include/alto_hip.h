@@ -192,13 +192,18 @@ int alto_coo_mttkrp(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64
  * memory and 64 GiB) and streams it in the mode's later inner iterations
  * instead of gathering those rows again (the reference's precomputed rows,
  * include/apr.hpp:1049-1165); 0 = gather them every time.
+ * ALTO_TUNE_LOCKSTEP_DENSE_BATCH: 1 (default) = alto_cpd_sessions_iterate
+ * at rank <= 64 runs each mode's R x R solves, row solves, normalisations and
+ * Gram updates of all its sessions in one launch per kernel, and their fits
+ * in one launch, on the caller's stream; 0 = one chain of launches per
+ * session over 4 internal streams.
  * Streams are re-ordered on their next use. */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
        ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9,
        ALTO_TUNE_NARROW_RANK = 10, ALTO_TUNE_PHI_VARIANT = 11, ALTO_TUNE_NARROW_SMALL = 12,
        ALTO_TUNE_FIBERS = 13, ALTO_TUNE_FIBER_MIN = 14, ALTO_TUNE_FIBER_CHUNK_LOG2 = 15,
        ALTO_TUNE_FIBER_MIN_NNZ = 16, ALTO_TUNE_FIBER_MODE_RULE = 17, ALTO_TUNE_DETERMINISTIC = 18,
-       ALTO_TUNE_APR_PI_CACHE = 19 };
+       ALTO_TUNE_APR_PI_CACHE = 19, ALTO_TUNE_LOCKSTEP_DENSE_BATCH = 20 };
 int alto_tune_set(int param, int value);
 
 /* Autotuner (no reference counterpart; the reference fixes its shapes at

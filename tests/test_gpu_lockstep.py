@@ -156,3 +156,69 @@ def test_lockstep_with_hot_rows():
         _compare(ts, alone, lock)
     finally:
         A.check(A.lib().alto_tune_set(7, 0), "tune_set")
+
+
+DET, DENSE_BATCH = 18, 20
+
+
+def _bit_identical(ts, alone, lock):
+    for (dims, _), a, b in zip(ts, alone, lock):
+        Ua, la, fa, _ = a.get()
+        Ub, lb, fb, _ = b.get()
+        assert fa == fb, (dims, fa, fb)
+        assert np.array_equal(la, lb), dims
+        assert all(np.array_equal(ua, ub) for ua, ub in zip(Ua, Ub)), dims
+
+
+def _det_lockstep(specs, R, prec, iters, seed0, dense_batch=1, eps=0.0):
+    """Sessions alone and in lockstep with deterministic reductions: the
+    batched MTTKRP sums in the same order as each tensor's own, and the
+    batched dense launches run the single-session kernels' bodies over the
+    same blocks, so the two agree bit for bit."""
+    A = _A()
+    lib = A.lib()
+    A.check(lib.alto_tune_set(DET, 1), "tune_set")
+    A.check(lib.alto_tune_set(DENSE_BATCH, dense_batch), "tune_set")
+    try:
+        ts = _tensors(specs, seed0=seed0)
+        alone, lock = [], []
+        for k, (dims, g) in enumerate(ts):
+            U, _ = A.kruskal_random_init(dims, R, seed0 + k)
+            alone.append(A.CpdSession(g, U, precision=prec))
+            lock.append(A.CpdSession(g, U, precision=prec))
+        want = [len(s.iterate(iters, eps)[0]) for s in alone]
+        assert A.cpd_sessions_iterate(lock, iters, eps) == want
+        _bit_identical(ts, alone, lock)
+    finally:
+        A.check(lib.alto_tune_set(DET, 0), "tune_set")
+        A.check(lib.alto_tune_set(DENSE_BATCH, 1), "tune_set")
+
+
+@pytest.mark.parametrize("R", [1, 7, 16, 33, 64, 65])
+def test_lockstep_batched_dense_bit_identical_fp64(R):
+    """Ranks through every lane shape of the batched dense kernels (1-64)
+    and one above them (65: the per-session chain)."""
+    _det_lockstep(SPECS, R, 0, 5, 60)
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+@pytest.mark.parametrize("R", [8, 33])
+def test_lockstep_batched_dense_bit_identical_reduced(prec, R):
+    _det_lockstep(SPECS, R, prec, 5, 70)
+
+
+def test_lockstep_per_session_chain_bit_identical():
+    """The per-session chains on the helper streams (batching off)."""
+    _det_lockstep(SPECS, 16, 0, 5, 80, dense_batch=0)
+
+
+def test_lockstep_batched_dense_many_sessions_and_convergence():
+    """20 sessions (two groups of batched launches) under a fit tolerance:
+    each stops where it would alone, with the same values."""
+    rng = np.random.default_rng(3)
+    specs = []
+    for k in range(20):
+        nm = 3 + k % 3
+        dims = [int(x) for x in rng.integers(10, 120, size=nm)]
+        specs.append((dims, int(rng.integers(500, 6000)), "zipf" if k % 2 else "uniform"))
+    _det_lockstep(specs, 12, 0, 40, 90, eps=1e-4)
