@@ -195,6 +195,8 @@ int fiber_rule();
 int det_enabled();
 // lockstep CP-ALS sessions' dense steps batched (ALTO_TUNE_LOCKSTEP_DENSE_BATCH)
 int lockstep_dense_batch_enabled();
+// ... and replayed as a HIP graph while the active set repeats (ALTO_TUNE_LOCKSTEP_GRAPHS)
+int lockstep_graphs_enabled();
 int apr_pi_cache_enabled();
 int mttkrp_det_device(alto_tensor* t, int mode, int R, const void* const* U, double* out, int precision, hipStream_t s);
 int phi_det_device(alto_tensor* t, int mode, int R, const double* const* U, double* phi, double eps, hipStream_t s);

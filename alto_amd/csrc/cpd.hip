@@ -738,12 +738,12 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
     if (dense_batched) ALTO_TRY(gathered.alloc((size_t)n * sizeof(double)));
     std::vector<DenseModeJob> djobs;
     std::vector<DenseFitJob> fjobs;
-    auto dense_groups = [&]() -> int {
+    auto dense_groups = [&](hipStream_t st) -> int {
         for (size_t k0 = 0; k0 < djobs.size(); k0 += DENSE_BATCH_MAX) {
             const int cnt = (int)std::min<size_t>(DENSE_BATCH_MAX, djobs.size() - k0);
-            if (prec == ALTO_PREC_F64) ALTO_TRY((dense_mode_batch<double, double>(cnt, &djobs[k0], R, true, s)));
-            else if (prec == ALTO_PREC_F32) ALTO_TRY((dense_mode_batch<float, float>(cnt, &djobs[k0], R, false, s)));
-            else ALTO_TRY((dense_mode_batch<float, bf16_t>(cnt, &djobs[k0], R, false, s)));
+            if (prec == ALTO_PREC_F64) ALTO_TRY((dense_mode_batch<double, double>(cnt, &djobs[k0], R, true, st)));
+            else if (prec == ALTO_PREC_F32) ALTO_TRY((dense_mode_batch<float, float>(cnt, &djobs[k0], R, false, st)));
+            else ALTO_TRY((dense_mode_batch<float, bf16_t>(cnt, &djobs[k0], R, false, st)));
         }
         return ALTO_OK;
     };
@@ -753,65 +753,129 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
     std::vector<const void* const*> fpp;
     std::vector<void*> outs;
     std::vector<int> act;
+    int maxN = 0;
+    // the batched MTTKRP of mode index j over the active sessions, on st
+    auto mttkrp_step = [&](int j, hipStream_t st) -> int {
+        ts.clear();
+        modes.clear();
+        fpp.clear();
+        outs.clear();
+        for (int i : act)
+            if (j < b[i]->nmodes()) {
+                void* M = nullptr;
+                b[i]->mode_io(j, fp[i].data(), &M);
+                ts.push_back(b[i]->t);
+                modes.push_back(j);
+                fpp.push_back(fp[i].data());
+                outs.push_back(M);
+            }
+        // M is zero here (cleared at set-up and by each mode's last launch)
+        return mttkrp_batched_impl((int)ts.size(), ts.data(), modes.data(), (uint64_t)R, fpp.data(), outs.data(),
+                                   prec, 1, st);
+    };
+    // one iteration of the active sessions with batched dense launches, on
+    // st; the fits land in fits[i]
+    auto batched_iteration = [&](hipStream_t st) -> int {
+        for (int j = 0; j < maxN; ++j) {
+            ALTO_TRY(mttkrp_step(j, st));
+            djobs.clear();
+            for (int i : act)
+                if (j < b[i]->nmodes()) {
+                    djobs.emplace_back();
+                    b[i]->mode_job(j, &djobs.back());
+                }
+            ALTO_TRY(dense_groups(st));
+        }
+        fjobs.clear();
+        for (int i : act) {
+            fjobs.emplace_back();
+            b[i]->fit_job(gathered.as<double>() + i, &fjobs.back());
+        }
+        for (size_t k0 = 0; k0 < fjobs.size(); k0 += DENSE_BATCH_MAX)
+            ALTO_TRY(fit_batch((int)std::min<size_t>(DENSE_BATCH_MAX, fjobs.size() - k0), &fjobs[k0], R, st));
+        HIP_TRY(hipMemcpyAsync(fits, gathered.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st));
+        return ALTO_OK;
+    };
+    // With ALTO_TUNE_LOCKSTEP_GRAPHS an iteration whose active sessions and
+    // first-iteration flags repeat the previous iteration's is captured as
+    // a HIP graph (on a private stream: `s` may be the null stream) and
+    // replayed while they stay the same.  The first iteration of a set runs
+    // eagerly, so buffers that a first use allocates exist before a capture.
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<int> sig;
+        ~StepGraph()
+        {
+            if (exec) (void)hipGraphExecDestroy(exec);
+        }
+    } graph;
+    static thread_local hipStream_t capture_stream[8] = {};
+    const bool use_graph = dense_batched && lockstep_graphs_enabled();
+    if (use_graph && !capture_stream[dev & 7])
+        HIP_TRY(hipStreamCreateWithFlags(&capture_stream[dev & 7], hipStreamNonBlocking));
+    std::vector<int> sig, prev_sig;
+    auto capture = [&]() -> int {
+        hipStream_t cs = capture_stream[dev & 7];
+        if (graph.exec) {
+            (void)hipGraphExecDestroy(graph.exec);
+            graph.exec = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int st = batched_iteration(cs);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(cs, &g);
+        if (st != ALTO_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (ec != hipSuccess) {
+            set_error_hip(ec, "hipStreamEndCapture (lockstep iteration)");
+            return ALTO_ERR_HIP;
+        }
+        const hipError_t ei = hipGraphInstantiate(&graph.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            graph.exec = nullptr;
+            set_error_hip(ei, "hipGraphInstantiate (lockstep iteration)");
+            return ALTO_ERR_HIP;
+        }
+        graph.sig = sig;
+        return ALTO_OK;
+    };
     for (int k = 0; k < max_iters; ++k) {
         act.clear();
-        int maxN = 0;
+        sig.clear();
+        maxN = 0;
         for (int i = 0; i < n; ++i)
             if (!b[i]->converged) {
                 act.push_back(i);
+                sig.push_back(2 * i + (b[i]->it == 0 ? 1 : 0));
                 maxN = std::max(maxN, b[i]->nmodes());
             }
         if (act.empty()) break;
         const double t0 = now_s();
-        for (int j = 0; j < maxN; ++j) {
-            ts.clear();
-            modes.clear();
-            fpp.clear();
-            outs.clear();
-            for (int i : act)
-                if (j < b[i]->nmodes()) {
-                    void* M = nullptr;
-                    b[i]->mode_io(j, fp[i].data(), &M);
-                    ts.push_back(b[i]->t);
-                    modes.push_back(j);
-                    fpp.push_back(fp[i].data());
-                    outs.push_back(M);
-                }
-            // M is zero here (cleared at set-up and by each mode's last launch)
-            ALTO_TRY(mttkrp_batched_impl((int)ts.size(), ts.data(), modes.data(), (uint64_t)b[0]->R, fpp.data(),
-                                         outs.data(), b[0]->prec, 1, s));
-            if (dense_batched) {
-                djobs.clear();
-                for (int i : act)
-                    if (j < b[i]->nmodes()) {
-                        djobs.emplace_back();
-                        b[i]->mode_job(j, &djobs.back());
-                    }
-                ALTO_TRY(dense_groups());
-                continue;
-            }
-            ALTO_TRY(fork());
-            int q = 0;
-            for (int i : act)
-                if (j < b[i]->nmodes()) ALTO_TRY(b[i]->mode_rest(j, lane(q++)));
-            ALTO_TRY(join());
-        }
-        if (dense_batched) {
-            fjobs.clear();
-            for (int i : act) {
-                fjobs.emplace_back();
-                b[i]->fit_job(gathered.as<double>() + i, &fjobs.back());
-            }
-            for (size_t k0 = 0; k0 < fjobs.size(); k0 += DENSE_BATCH_MAX)
-                ALTO_TRY(fit_batch((int)std::min<size_t>(DENSE_BATCH_MAX, fjobs.size() - k0), &fjobs[k0], R, s));
-            HIP_TRY(hipMemcpyAsync(fits, gathered.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s));
-            for (int i : act) ALTO_TRY(b[i]->fence(s));
+        if (use_graph && sig == prev_sig) {
+            if (!graph.exec || graph.sig != sig) ALTO_TRY(capture());
+            HIP_TRY(hipGraphLaunch(graph.exec, s));
+        } else if (dense_batched) {
+            ALTO_TRY(batched_iteration(s));
         } else {
+            for (int j = 0; j < maxN; ++j) {
+                ALTO_TRY(mttkrp_step(j, s));
+                ALTO_TRY(fork());
+                int q = 0;
+                for (int i : act)
+                    if (j < b[i]->nmodes()) ALTO_TRY(b[i]->mode_rest(j, lane(q++)));
+                ALTO_TRY(join());
+            }
             ALTO_TRY(fork());
             int q = 0;
             for (int i : act) ALTO_TRY(b[i]->fit_launch(lane(q++), fits + i));
             ALTO_TRY(join());
         }
+        prev_sig = sig;
+        if (dense_batched)
+            for (int i : act) ALTO_TRY(b[i]->fence(s));
         HIP_TRY(hipStreamSynchronize(s));
         const double dt = now_s() - t0;
         for (int i : act) {

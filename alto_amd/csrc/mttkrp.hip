@@ -94,6 +94,9 @@ int graphs_enabled() { return g_graphs; }
 // lockstep CP-ALS: the sessions' mode updates and fits in batched launches
 static int g_lockstep_dense = 1;
 int lockstep_dense_batch_enabled() { return g_lockstep_dense; }
+// lockstep CP-ALS: repeated iterations replayed as a HIP graph
+static int g_lockstep_graphs = 0;
+int lockstep_graphs_enabled() { return g_lockstep_graphs; }
 int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
@@ -699,6 +702,10 @@ int alto_tune_set(int param, int value)
     }
     if (param == ALTO_TUNE_DETERMINISTIC) {
         g_det = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_LOCKSTEP_GRAPHS) {
+        g_lockstep_graphs = value ? 1 : 0;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_LOCKSTEP_DENSE_BATCH) {

@@ -775,6 +775,7 @@ def batched_leg(n_jobs=16, reps=50):
         alone = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
         lock = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
         chains = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
+        graphed = [A.CpdSession(t, U, precision=prec) for (t, _, _), U in zip(tensors, inits)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for x in alone:
@@ -792,15 +793,23 @@ def batched_leg(n_jobs=16, reps=50):
             t4 = time.perf_counter()
         finally:
             A.check(L.alto_tune_set(20, 1), "tune_set")
-        dfit = max(max(abs(x.get()[2] - y.get()[2]), abs(x.get()[2] - z.get()[2]))
-                   for x, y, z in zip(alone, lock, chains))
+        A.check(L.alto_tune_set(21, 1), "tune_set")  # ALTO_TUNE_LOCKSTEP_GRAPHS
+        try:
+            t5 = time.perf_counter()
+            A.cpd_sessions_iterate(graphed, 20)
+            torch.cuda.synchronize()
+            t6 = time.perf_counter()
+        finally:
+            A.check(L.alto_tune_set(21, 0), "tune_set")
+        dfit = max(max(abs(x.get()[2] - y.get()[2]), abs(x.get()[2] - z.get()[2]), abs(x.get()[2] - w.get()[2]))
+                   for x, y, z, w in zip(alone, lock, chains, graphed))
         out[name] = {"iterations": 20, "one_after_another_ms": (t1 - t0) * 1e3, "lockstep_ms": (t2 - t1) * 1e3,
-                     "lockstep_per_session_chains_ms": (t4 - t3) * 1e3,
+                     "lockstep_per_session_chains_ms": (t4 - t3) * 1e3, "lockstep_graph_replay_ms": (t6 - t5) * 1e3,
                      "speedup": (t1 - t0) / (t2 - t1), "max_fit_delta": dfit}
         log(f"lockstep CP-ALS {name}: {n_jobs} sessions x 20 iterations, one after another {(t1 - t0) * 1e3:.1f} ms, "
             f"lockstep {(t2 - t1) * 1e3:.1f} ms (x{(t1 - t0) / (t2 - t1):.2f}; with per-session dense chains "
-            f"{(t4 - t3) * 1e3:.1f} ms), max fit delta {dfit:.1e}")
-        del alone, lock, chains
+            f"{(t4 - t3) * 1e3:.1f} ms; replayed as a graph {(t6 - t5) * 1e3:.1f} ms), max fit delta {dfit:.1e}")
+        del alone, lock, chains, graphed
     del tensors
     torch.cuda.empty_cache()
     return out

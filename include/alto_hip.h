@@ -196,14 +196,17 @@ int alto_coo_mttkrp(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64
  * at rank <= 64 runs each mode's R x R solves, row solves, normalisations and
  * Gram updates of all its sessions in one launch per kernel, and their fits
  * in one launch, on the caller's stream; 0 = one chain of launches per
- * session over 4 internal streams.
+ * session over 4 internal streams.  ALTO_TUNE_LOCKSTEP_GRAPHS: 1 = with the
+ * batched launches, an iteration whose active sessions repeat the previous
+ * iteration's is captured once as a HIP graph and replayed (0, default:
+ * eager launches).
  * Streams are re-ordered on their next use. */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
        ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9,
        ALTO_TUNE_NARROW_RANK = 10, ALTO_TUNE_PHI_VARIANT = 11, ALTO_TUNE_NARROW_SMALL = 12,
        ALTO_TUNE_FIBERS = 13, ALTO_TUNE_FIBER_MIN = 14, ALTO_TUNE_FIBER_CHUNK_LOG2 = 15,
        ALTO_TUNE_FIBER_MIN_NNZ = 16, ALTO_TUNE_FIBER_MODE_RULE = 17, ALTO_TUNE_DETERMINISTIC = 18,
-       ALTO_TUNE_APR_PI_CACHE = 19, ALTO_TUNE_LOCKSTEP_DENSE_BATCH = 20 };
+       ALTO_TUNE_APR_PI_CACHE = 19, ALTO_TUNE_LOCKSTEP_DENSE_BATCH = 20, ALTO_TUNE_LOCKSTEP_GRAPHS = 21 };
 int alto_tune_set(int param, int value);
 
 /* Autotuner (no reference counterpart; the reference fixes its shapes at

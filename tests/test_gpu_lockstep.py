@@ -158,7 +158,7 @@ def test_lockstep_with_hot_rows():
         A.check(A.lib().alto_tune_set(7, 0), "tune_set")
 
 
-DET, DENSE_BATCH = 18, 20
+DET, DENSE_BATCH, LOCK_GRAPHS = 18, 20, 21
 
 
 def _bit_identical(ts, alone, lock):
@@ -170,15 +170,17 @@ def _bit_identical(ts, alone, lock):
         assert all(np.array_equal(ua, ub) for ua, ub in zip(Ua, Ub)), dims
 
 
-def _det_lockstep(specs, R, prec, iters, seed0, dense_batch=1, eps=0.0):
+def _det_lockstep(specs, R, prec, iters, seed0, dense_batch=1, eps=0.0, graphs=0):
     """Sessions alone and in lockstep with deterministic reductions: the
     batched MTTKRP sums in the same order as each tensor's own, and the
     batched dense launches run the single-session kernels' bodies over the
-    same blocks, so the two agree bit for bit."""
+    same blocks, so the two agree bit for bit (graphs: repeated iterations
+    replayed from a captured graph)."""
     A = _A()
     lib = A.lib()
     A.check(lib.alto_tune_set(DET, 1), "tune_set")
     A.check(lib.alto_tune_set(DENSE_BATCH, dense_batch), "tune_set")
+    A.check(lib.alto_tune_set(LOCK_GRAPHS, graphs), "tune_set")
     try:
         ts = _tensors(specs, seed0=seed0)
         alone, lock = [], []
@@ -192,6 +194,7 @@ def _det_lockstep(specs, R, prec, iters, seed0, dense_batch=1, eps=0.0):
     finally:
         A.check(lib.alto_tune_set(DET, 0), "tune_set")
         A.check(lib.alto_tune_set(DENSE_BATCH, 1), "tune_set")
+        A.check(lib.alto_tune_set(LOCK_GRAPHS, 0), "tune_set")
 
 
 @pytest.mark.parametrize("R", [1, 7, 16, 33, 64, 65])
@@ -222,3 +225,46 @@ def test_lockstep_batched_dense_many_sessions_and_convergence():
         dims = [int(x) for x in rng.integers(10, 120, size=nm)]
         specs.append((dims, int(rng.integers(500, 6000)), "zipf" if k % 2 else "uniform"))
     _det_lockstep(specs, 12, 0, 40, 90, eps=1e-4)
+
+
+@pytest.mark.parametrize("prec,R", [(0, 16), (0, 64), (1, 33), (2, 8)])
+def test_lockstep_graph_replays_bit_identical(prec, R):
+    """Iterations replayed from the captured graph (from the third on) equal
+    each session's eager run bit for bit."""
+    _det_lockstep(SPECS, R, prec, 8, 100, graphs=1)
+
+
+def test_lockstep_graph_recaptured_as_sessions_converge():
+    """Sessions that converge leave the set: the next iterations run eagerly
+    once and are captured again; every session stops where it would alone
+    (each stop depends on the fits of every replayed iteration) with the
+    same values."""
+    rng = np.random.default_rng(5)
+    specs = []
+    for k in range(18):
+        dims = [int(x) for x in rng.integers(10, 90, size=3 + k % 3)]
+        specs.append((dims, int(rng.integers(400, 4000)), "zipf" if k % 2 else "uniform"))
+    _det_lockstep(specs, 10, 0, 50, 110, eps=1e-4, graphs=1)
+
+
+@pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5)])
+def test_lockstep_graph_with_atomic_reductions(prec, tol):
+    """Graph replays with the default (atomic) reductions stay within the
+    stated tolerance of each session alone."""
+    A = _A()
+    lib = A.lib()
+    A.check(lib.alto_tune_set(LOCK_GRAPHS, 1), "tune_set")
+    try:
+        ts = _tensors(SPECS, seed0=120)
+        alone, lock = [], []
+        for k, (dims, g) in enumerate(ts):
+            U, _ = A.kruskal_random_init(dims, 16, 120 + k)
+            alone.append(A.CpdSession(g, U, precision=prec))
+            lock.append(A.CpdSession(g, U, precision=prec))
+        for x in alone:
+            x.iterate(10)
+        assert A.cpd_sessions_iterate(lock, 10) == [10] * len(ts)
+        for (dims, _), a, b in zip(ts, alone, lock):
+            assert abs(a.get()[2] - b.get()[2]) <= tol, dims
+    finally:
+        A.check(lib.alto_tune_set(LOCK_GRAPHS, 0), "tune_set")
