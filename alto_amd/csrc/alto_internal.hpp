@@ -203,5 +203,13 @@ int phi_det_device(alto_tensor* t, int mode, int R, const double* const* U, doub
 
 // cpd.hip: per-phase event timing of a session's eager iterations (default on)
 int cpd_session_set_timing(alto_cpd_session_t* sess, int on);
+// the stream a session's later work goes to; waits for the session's
+// current stream first
+int cpd_session_set_stream(alto_cpd_session_t* sess, hipStream_t s);
+// iterations run so far and whether the fit tolerance stopped the session
+int cpd_session_state(const alto_cpd_session_t* sess, int* iterations, int* converged);
+// server: jobs of at most *max_nnz nonzeros (0: none) iterated in lockstep
+// groups, at most *max_sessions of them at once (ALTO_TUNE_SERVE_LOCKSTEP_*)
+void serve_lockstep_settings(uint64_t* max_nnz, int* max_sessions);
 
 }  // namespace alto

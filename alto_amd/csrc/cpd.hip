@@ -603,6 +603,24 @@ int alto::cpd_session_set_timing(alto_cpd_session_t* sess, int on)
     return ALTO_OK;
 }
 
+int alto::cpd_session_set_stream(alto_cpd_session_t* sess, hipStream_t s)
+{
+    if (!sess) return ALTO_ERR_ARG;
+    auto* b = reinterpret_cast<alto::CpdSessionBase*>(sess);
+    HIP_TRY(hipStreamSynchronize(b->s));
+    b->s = s;
+    return ALTO_OK;
+}
+
+int alto::cpd_session_state(const alto_cpd_session_t* sess, int* iterations, int* converged)
+{
+    if (!sess) return ALTO_ERR_ARG;
+    const auto* b = reinterpret_cast<const alto::CpdSessionBase*>(sess);
+    if (iterations) *iterations = b->it;
+    if (converged) *converged = b->converged ? 1 : 0;
+    return ALTO_OK;
+}
+
 using namespace alto;
 
 extern "C" {
@@ -686,12 +704,22 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
     if (n == 0) return ALTO_OK;
     hipStream_t s = (hipStream_t)stream;
     for (auto* x : b) HIP_TRY(hipStreamSynchronize(x->s));  // earlier work of each session
-    double* fits = nullptr;
-    HIP_TRY(hipHostMalloc((void**)&fits, sizeof(double) * n, hipHostMallocDefault));
-    struct FitsGuard {
-        double* p;
-        ~FitsGuard() { (void)hipHostFree(p); }
-    } guard{fits};
+    // pinned fits, one buffer per thread, grown as needed and kept until the
+    // process ends (callers that step their sessions one iteration per call
+    // make many calls)
+    struct Pinned {
+        double* p = nullptr;
+        int cap = 0;
+    };
+    static thread_local Pinned pinned;
+    if (pinned.cap < n) {
+        if (pinned.p) HIP_TRY(hipHostFree(pinned.p));
+        pinned.p = nullptr;
+        pinned.cap = 0;
+        HIP_TRY(hipHostMalloc((void**)&pinned.p, sizeof(double) * (size_t)std::max(n, 64), hipHostMallocDefault));
+        pinned.cap = std::max(n, 64);
+    }
+    double* fits = pinned.p;
     // the sessions' solve chains are independent: they run on NAUX private
     // streams, forked from `s` after each batched MTTKRP and joined back
     // before the next one (the next MTTKRP accumulates into the M that each

@@ -97,6 +97,14 @@ int lockstep_dense_batch_enabled() { return g_lockstep_dense; }
 // lockstep CP-ALS: repeated iterations replayed as a HIP graph
 static int g_lockstep_graphs = 0;
 int lockstep_graphs_enabled() { return g_lockstep_graphs; }
+// server: small jobs iterated in lockstep groups
+static uint64_t g_serve_lock_nnz = 0;
+static int g_serve_lock_sessions = 64;
+void serve_lockstep_settings(uint64_t* max_nnz, int* max_sessions)
+{
+    *max_nnz = g_serve_lock_nnz;
+    *max_sessions = g_serve_lock_sessions;
+}
 int hot_rows_enabled() { return g_hot; }
 int tuned_chunk_log2() { return g_chunk; }
 int set_chunk(int v)
@@ -702,6 +710,16 @@ int alto_tune_set(int param, int value)
     }
     if (param == ALTO_TUNE_DETERMINISTIC) {
         g_det = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_SERVE_LOCKSTEP_NNZ) {
+        if (value < 0) return ALTO_ERR_ARG;
+        g_serve_lock_nnz = (uint64_t)value;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_SERVE_LOCKSTEP_SESSIONS) {
+        if (value < 1 || value > 4096) return ALTO_ERR_ARG;
+        g_serve_lock_sessions = value;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_LOCKSTEP_GRAPHS) {

@@ -49,12 +49,21 @@ def _bind():
 
 
 class Server:
-    """Continuous-batching server: admission by HBM footprint, one stream per slot."""
+    """Continuous-batching server: admission by HBM footprint, one stream per
+    slot; with lockstep_nnz > 0, jobs of at most that many nonzeros (rank <=
+    64) are iterated in lockstep groups, at most lockstep_sessions at once."""
 
-    def __init__(self, hbm_budget_bytes=200 << 30, slots=8):
+    def __init__(self, hbm_budget_bytes=200 << 30, slots=8, lockstep_nnz=0, lockstep_sessions=64):
         self.L = _bind()
         h = C.c_void_p()
-        check(self.L.alto_server_create(int(hbm_budget_bytes), int(slots), C.byref(h)), "server_create")
+        # the server reads these settings when it is created
+        check(self.L.alto_tune_set(22, int(lockstep_nnz)), "tune_set")
+        check(self.L.alto_tune_set(23, int(lockstep_sessions)), "tune_set")
+        try:
+            check(self.L.alto_server_create(int(hbm_budget_bytes), int(slots), C.byref(h)), "server_create")
+        finally:
+            self.L.alto_tune_set(22, 0)
+            self.L.alto_tune_set(23, 64)
         self.h = h.value
         self._keep = {}
 
@@ -149,34 +158,43 @@ def materialize(jobs, device="cuda"):
     return jobs
 
 
-def run_stream(jobs, precision, slots, budget=200 << 30, max_iters=20, epsilon=1e-5, time_scale=1.0, window=48):
+def run_stream(jobs, precision, slots, budget=200 << 30, max_iters=20, epsilon=1e-5, time_scale=1.0, window=48,
+               lockstep_nnz=0):
     """Submits the jobs at their (scaled) arrival times and returns throughput
     and latency.  A job without a materialised tensor ("coords") has it
     generated on the GPU just before its submission and freed as soon as the
     job finished, and at most `window` generated jobs hold their inputs at
-    once (submission waits for the oldest to finish), so a stream of any
+    once (submission waits for any of them to finish), so a stream of any
     length runs in bounded memory.  Latency is submission to result; the
-    generation, before submission, is not part of it."""
+    generation, before submission, is not part of it.  lockstep_nnz: see
+    Server."""
     import collections
 
     import torch
 
-    srv = Server(budget, slots)
+    srv = Server(budget, slots, lockstep_nnz=lockstep_nnz)
     ids, nnzs = [], []
     live = collections.deque()  # (job id, generated inputs) in submission order
     done = {}
 
     def retire(block):
-        # frees the inputs of finished jobs, oldest first; with block, waits
-        # for the oldest one
+        # frees the inputs of every finished job; with block, waits until at
+        # least one has finished (whichever it is: a long job submitted
+        # early does not hold the window)
         while live:
-            jid, _ = live[0]
-            r, _, _ = srv.result(jid, wait=block)
-            if r.status == 0 or r.status == 1:
+            keep = collections.deque()
+            for jid, inputs in live:
+                r, _, _ = srv.result(jid, wait=False)
+                if r.status == 0 or r.status == 1:
+                    keep.append((jid, inputs))
+                else:
+                    done[jid] = r
+            freed = len(keep) < len(live)
+            live.clear()
+            live.extend(keep)
+            if freed or not block:
                 return
-            done[jid] = r
-            live.popleft()
-            block = False
+            time.sleep(0.0002)
 
     t0 = time.perf_counter()
     for j in jobs:
@@ -258,12 +276,14 @@ def main(argv=None):
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--sequential", action="store_true", help="no server: jobs one after another")
     ap.add_argument("--time-scale", type=float, default=0.0, help="arrival-time scale (0: all jobs at once)")
+    ap.add_argument("--lockstep-nnz", type=int, default=0,
+                    help="jobs of at most this many nonzeros iterated in lockstep groups (0: none)")
     args = ap.parse_args(argv)
     jobs = materialize(job_stream(args.jobs, seed=args.seed))
     if args.sequential:
         r = run_sequential(jobs, args.precision)
     else:
-        r = run_stream(jobs, args.precision, args.slots, time_scale=args.time_scale)
+        r = run_stream(jobs, args.precision, args.slots, time_scale=args.time_scale, lockstep_nnz=args.lockstep_nnz)
     r.pop("fits", None)
     print(json.dumps({"precision": args.precision, "sequential": args.sequential, "slots": args.slots, **r}),
           flush=True)

@@ -22,8 +22,10 @@ def _jobs(n, seed=0):
     return out
 
 
+@pytest.mark.parametrize("lockstep_nnz", [0, 15000, 1 << 30])
 @pytest.mark.parametrize("prec,tol", [(0, 1e-10), (1, 1e-5), (2, 1e-3)])
-def test_alone_equals_in_mix(prec, tol):
+def test_alone_equals_in_mix(prec, tol, lockstep_nnz):
+    """lockstep_nnz: none, part or all of the mix iterated in lockstep groups."""
     from alto_amd.serve import Server
     jobs = _jobs(12)
     alone = []
@@ -34,7 +36,7 @@ def test_alone_equals_in_mix(prec, tol):
         assert r.status == 2
         alone.append((r.fit, r.iters, U))
     s1.close()
-    s8 = Server(8 << 30, 8)
+    s8 = Server(8 << 30, 8, lockstep_nnz=lockstep_nnz)
     ids = [s8.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], 15, 1e-5, j["seed"], prec, want_factors=True)
            for j in jobs]
     for j, jid, (fit, iters, U) in zip(jobs, ids, alone):
@@ -83,3 +85,33 @@ def test_sequential_runner_equals_server():
     s4.close()
     assert seq["jobs"] == 10
     np.testing.assert_allclose(seq["fits"], fits, rtol=1e-10, atol=1e-12)
+
+
+def test_lockstep_groups_with_mixed_limits():
+    """Jobs of several ranks (64 joins a group, 72 does not), tolerances and
+    iteration limits through the lockstep lane, with fewer lockstep slots
+    than jobs: each stops where it stops alone, with the same fit."""
+    from alto_amd.serve import Server
+    rng = np.random.default_rng(21)
+    jobs = []
+    for k in range(20):
+        dims = [int(rng.integers(30, 200)) for _ in range(int(rng.integers(3, 6)))]
+        cidx, vals = zipf_tensor(dims, int(rng.integers(2000, 20000)), 300 + k)
+        jobs.append(dict(dims=dims, cidx=cidx, vals=vals, rank=int(rng.choice([8, 16, 64, 72])), seed=40 + k,
+                         iters=int(rng.integers(3, 25)), eps=float(rng.choice([0.0, 1e-4, 1e-6]))))
+    alone = []
+    s1 = Server(8 << 30, 1)
+    for j in jobs:
+        jid = s1.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], j["iters"], j["eps"], j["seed"])
+        r, _, _ = s1.result(jid, True)
+        assert r.status == 2
+        alone.append(r)
+    s1.close()
+    s = Server(8 << 30, 4, lockstep_nnz=1 << 30, lockstep_sessions=6)
+    ids = [s.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], j["iters"], j["eps"], j["seed"]) for j in jobs]
+    for j, jid, a in zip(jobs, ids, alone):
+        r, _, _ = s.result(jid, True)
+        assert r.status == 2
+        assert r.iters == a.iters, (j["rank"], j["iters"], j["eps"])
+        assert abs(r.fit - a.fit) <= 1e-10 * max(1.0, abs(a.fit))
+    s.close()
