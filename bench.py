@@ -322,11 +322,21 @@ def main():
                               "fit": float(vfits[-1]), "fit_delta_vs_fp64_gpu": float(abs(vfits[-1] - fits[-1]))}
             log(f"{name} CP-ALS: {vdt / args.steps * 1e3:.3f} ms/iter, fit {vfits[-1]:.6f}")
 
-    cpu = fit1 = None
+    cpu = fit1 = fitn = None
     if not args.no_cpu:
-        cpu, fit1_oracle = cpu_baseline(T, U0)
+        n_total = args.warmup + args.steps
+        cpu, fit1_oracle, ofits = cpu_baseline(T, U0, n_total)
         fit1 = {"gpu": fit1_gpu, "oracle": fit1_oracle, "abs_diff": abs(fit1_gpu - fit1_oracle)}
         log(f"iteration-1 fit: GPU {fit1_gpu:.15f} oracle {fit1_oracle:.15f} (|diff| {fit1['abs_diff']:.1e})")
+        # the fit after the runs' warmup + timed iterations against the
+        # oracle's FP64 run from the same model
+        fitn = {"iterations": n_total, "oracle": float(ofits[-1]), "fp64_gpu": float(fits[-1]),
+                "fp64_abs_diff": abs(float(fits[-1]) - float(ofits[-1]))}
+        for name, v in variants.items():
+            v["fit_delta_vs_oracle"] = abs(v["fit"] - float(ofits[-1]))
+            fitn[name + "_abs_diff"] = v["fit_delta_vs_oracle"]
+        log(f"fit after {n_total} iterations vs the oracle: " + ", ".join(
+            f"{k} {v:.1e}" for k, v in fitn.items() if k.endswith("abs_diff")))
     del T
     torch.cuda.empty_cache()
 
@@ -376,6 +386,7 @@ def main():
         "device_bytes": {"format": bytes_format, "with_mode_streams": bytes_streams},
         "cpd_timers_s": tm,
         "iteration1_fit_vs_oracle": fit1,
+        "final_fit_vs_oracle": fitn,
         "precision_variants": variants,
         "c1_cpu_config": c1,
         "cp_apr": apr,
@@ -844,10 +855,11 @@ class _HostStdoutToNull:
         return False
 
 
-def cpu_baseline(T, U0):
+def cpu_baseline(T, U0, n_total):
     """One CP-ALS iteration on the host cores (this process's CPU share), on
     the same tensor exported from the GPU; returns (baseline, the
-    iteration's fit).  With the reference harness built (oracle/_ref: the
+    iteration's fit, the oracle's fit trace over n_total iterations from the
+    same model -- a parity check, outside the timing).  With the reference harness built (oracle/_ref: the
     reference's include/alto.hpp compiled from its sources), the iteration's
     MTTKRPs are the reference's own mttkrp_alto_par and the rest (R x R
     solves, norms, Grams, fit: the reference's driver needs BLAS / LAPACK,
@@ -874,9 +886,12 @@ def cpu_baseline(T, U0):
             "host_physical_cores": cores, "cpu_model": model,
             "sample": f"1 CP-ALS iteration (rank {RANK}) on the full {inf.nnz}-nonzero tensor, "
                       f"OpenMP over {threads} threads (this process's CPU share), nprtn {T.nprtn}"}
+    t4 = time.time()
+    _, ofits = o.cpd_als([u.copy() for u in U0], np.ones(RANK), n_total, 0.0)
+    log(f"CPU oracle: {n_total} CP-ALS iterations for the fit check in {time.time() - t4:.1f}s")
     W = st["idx"].shape[1]
     if not oracle.ref_available(W):
-        return port, float(fit[0])
+        return port, float(fit[0]), ofits
     t2 = time.time()
     for m in range(3):
         o.mttkrp(m, U)
@@ -900,7 +915,7 @@ def cpu_baseline(T, U0):
                       f"threads, nprtn {T.nprtn}: the reference's mttkrp_alto_par for the three MTTKRPs "
                       f"({t_ref_mttkrp:.3f} s) plus the oracle's solves, norms, Grams and fit "
                       f"({max(dt - t_oracle_mttkrp, 0.0):.3f} s)",
-            "reference_allmode_mttkrp_s": t_ref_mttkrp, "port": port}, float(fit[0])
+            "reference_allmode_mttkrp_s": t_ref_mttkrp, "port": port}, float(fit[0]), ofits
 
 
 if __name__ == "__main__":
