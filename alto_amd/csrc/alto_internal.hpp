@@ -195,6 +195,8 @@ int fiber_rule();
 int det_enabled();
 // lockstep CP-ALS sessions' dense steps batched (ALTO_TUNE_LOCKSTEP_DENSE_BATCH)
 int lockstep_dense_batch_enabled();
+// FP32 / BF16 CP-ALS Grams and row solves on FP64 MFMA (ALTO_TUNE_DENSE_F64)
+int dense_f64_enabled();
 // ... and replayed as a HIP graph while the active set repeats (ALTO_TUNE_LOCKSTEP_GRAPHS)
 int lockstep_graphs_enabled();
 int apr_pi_cache_enabled();

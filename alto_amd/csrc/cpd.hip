@@ -125,7 +125,9 @@ static int cpd_alloc(alto_tensor* t, int R, CpdState<UT, ST, MT>& c)
     return ALTO_OK;
 }
 
-static bool gram_f64(int prec) { return prec == ALTO_PREC_F64; }
+// Grams on FP64 MFMA for FP64 models and, with ALTO_TUNE_DENSE_F64, for
+// FP32 / BF16 models (from their stored factors)
+static bool gram_f64(int prec) { return prec == ALTO_PREC_F64 || dense_f64_enabled(); }
 
 template <typename UT, typename ST, typename MT>
 static int cpd_upload(alto_tensor* t, CpdState<UT, ST, MT>& c, double* const* Uh, hipStream_t s,
@@ -770,8 +772,9 @@ int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int ma
         for (size_t k0 = 0; k0 < djobs.size(); k0 += DENSE_BATCH_MAX) {
             const int cnt = (int)std::min<size_t>(DENSE_BATCH_MAX, djobs.size() - k0);
             if (prec == ALTO_PREC_F64) ALTO_TRY((dense_mode_batch<double, double>(cnt, &djobs[k0], R, true, st)));
-            else if (prec == ALTO_PREC_F32) ALTO_TRY((dense_mode_batch<float, float>(cnt, &djobs[k0], R, false, st)));
-            else ALTO_TRY((dense_mode_batch<float, bf16_t>(cnt, &djobs[k0], R, false, st)));
+            else if (prec == ALTO_PREC_F32)
+                ALTO_TRY((dense_mode_batch<float, float>(cnt, &djobs[k0], R, gram_f64(prec), st)));
+            else ALTO_TRY((dense_mode_batch<float, bf16_t>(cnt, &djobs[k0], R, gram_f64(prec), st)));
         }
         return ALTO_OK;
     };

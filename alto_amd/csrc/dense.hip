@@ -651,15 +651,15 @@ __device__ __attribute__((noinline)) void pinv_body(const double* __restrict__ g
 // U = lambda * U_normalised it equals sum_c lambda_c <U_normalised[:,c], M[:,c]>).
 // The 16 x R block of M is loaded into registers before the products, so
 // its loads are in flight together.  T16: 16-column tiles (R <= 16 T16).
-template <typename MT, typename UT, int T16>
+template <typename MT, typename UT, int T16, bool LOWP>
 __device__ __forceinline__ void apply_body(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ Wg,
                                            UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
                                            double* __restrict__ pin, uint32_t bid, uint32_t nblk)
 {
-    // FP32 / BF16 models (U in FP32; M in FP64 or FP32): the products on
-    // FP32 MFMA with M and the FP64 inverse rounded to FP32; FP64 models:
-    // FP64 MFMA
-    constexpr bool F32 = std::is_same<UT, float>::value;
+    // LOWP (FP32 / BF16 models with ALTO_TUNE_DENSE_F64 off): the products
+    // on FP32 MFMA with M and the FP64 inverse rounded to FP32; otherwise
+    // FP64 MFMA, the result rounded to U's type when it is stored
+    constexpr bool F32 = LOWP;
     using mma_t = typename std::conditional<F32, float, double>::type;
     using acc_t = typename std::conditional<F32, f4_t, d4_t>::type;
     constexpr int KS = 4 * T16;  // k-steps of 4 covering 16 T16 >= R columns
@@ -757,12 +757,12 @@ __device__ __forceinline__ void apply_body(const MT* __restrict__ M, uint64_t I,
     }
 }
 
-template <typename MT, typename UT, int T16>
+template <typename MT, typename UT, int T16, bool LOWP>
 __global__ void __launch_bounds__(256) k_apply(const MT* __restrict__ M, uint64_t I, int R, const double* __restrict__ Wg,
                                                UT* __restrict__ U, double* __restrict__ psum, double* __restrict__ pmax,
                                                double* __restrict__ pin)
 {
-    apply_body<MT, UT, T16>(M, I, R, Wg, U, psum, pmax, pin, blockIdx.x, gridDim.x);
+    apply_body<MT, UT, T16, LOWP>(M, I, R, Wg, U, psum, pmax, pin, blockIdx.x, gridDim.x);
 }
 
 // lambda: iteration 0 -> 2-norm of each column; later -> max(1, max_i U[i,r]).
@@ -956,7 +956,16 @@ int solve_device(const MT* M, uint64_t I, int R, const double* L, const double* 
     (void)L;
     (void)status;
     const size_t lds = (size_t)R * (R + 2) * 8;
-#define AP_(TT) hipLaunchKernelGGL((k_apply<MT, UT, TT>), dim3(nb), dim3(256), lds, s, M, I, R, P, U, psum, pmax, pin)
+    const bool lowp = std::is_same<UT, float>::value && !dense_f64_enabled();
+#define AP_(TT)                                                                                                     \
+    do {                                                                                                            \
+        if (lowp)                                                                                                   \
+            hipLaunchKernelGGL((k_apply<MT, UT, TT, true>), dim3(nb), dim3(256), lds, s, M, I, R, P, U, psum, pmax, \
+                               pin);                                                                                \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_apply<MT, UT, TT, false>), dim3(nb), dim3(256), lds, s, M, I, R, P, U, psum,     \
+                               pmax, pin);                                                                          \
+    } while (0)
     switch ((R + 15) / 16) {
     case 1: AP_(1); break;
     case 2: AP_(2); break;
@@ -1103,13 +1112,13 @@ __global__ void __launch_bounds__(BT) k_spd_inverse_batch(BatchTab<SpdJob> t, in
     spd_inverse_body<BT, NPT>(j.grams, j.nmode, j.mode, R, j.P, j.status, j.wsA, j.wsQ);
 }
 
-template <typename MT, typename UT, int T16>
+template <typename MT, typename UT, int T16, bool LOWP>
 __global__ void __launch_bounds__(256) k_apply_batch(BatchTab<ApplyJob> t, int R)
 {
     const int k = batch_job(t, blockIdx.x);
     const ApplyJob& j = t.a[k];
-    apply_body<MT, UT, T16>(j.M, j.I, R, j.P, (UT*)j.U, j.psum, j.pmax, j.pin, blockIdx.x - t.start[k],
-                            t.start[k + 1] - t.start[k]);
+    apply_body<MT, UT, T16, LOWP>(j.M, j.I, R, j.P, (UT*)j.U, j.psum, j.pmax, j.pin, blockIdx.x - t.start[k],
+                                  t.start[k + 1] - t.start[k]);
 }
 
 __global__ void __launch_bounds__(256) k_lambda_batch(BatchTab<LambdaJob> t, int R)
@@ -1181,7 +1190,15 @@ int dense_mode_batch(int n, const DenseModeJob* m, int R, bool f64, hipStream_t 
     else SPD_(1024, 4);
 #undef SPD_
     const size_t lds_ap = (size_t)R * (R + 2) * 8;
-#define AP_(TT) hipLaunchKernelGGL((k_apply_batch<double, UT, TT>), dim3(ap.start[n]), dim3(256), lds_ap, s, ap, R)
+    const bool lowp = std::is_same<UT, float>::value && !dense_f64_enabled();
+#define AP_(TT)                                                                                                  \
+    do {                                                                                                         \
+        if (lowp)                                                                                                \
+            hipLaunchKernelGGL((k_apply_batch<double, UT, TT, true>), dim3(ap.start[n]), dim3(256), lds_ap, s, ap, R); \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_apply_batch<double, UT, TT, false>), dim3(ap.start[n]), dim3(256), lds_ap, s, ap, \
+                               R);                                                                               \
+    } while (0)
     switch ((R + 15) / 16) {
     case 1: AP_(1); break;
     case 2: AP_(2); break;

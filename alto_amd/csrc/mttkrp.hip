@@ -97,6 +97,9 @@ int lockstep_dense_batch_enabled() { return g_lockstep_dense; }
 // lockstep CP-ALS: repeated iterations replayed as a HIP graph
 static int g_lockstep_graphs = 0;
 int lockstep_graphs_enabled() { return g_lockstep_graphs; }
+// FP32 / BF16 CP-ALS: Grams and row solves on FP64 MFMA
+static int g_dense_f64 = 1;
+int dense_f64_enabled() { return g_dense_f64; }
 // server: small jobs iterated in lockstep groups
 static uint64_t g_serve_lock_nnz = 0;
 static int g_serve_lock_sessions = 64;
@@ -710,6 +713,10 @@ int alto_tune_set(int param, int value)
     }
     if (param == ALTO_TUNE_DETERMINISTIC) {
         g_det = value ? 1 : 0;
+        return ALTO_OK;
+    }
+    if (param == ALTO_TUNE_DENSE_F64) {
+        g_dense_f64 = value ? 1 : 0;
         return ALTO_OK;
     }
     if (param == ALTO_TUNE_SERVE_LOCKSTEP_NNZ) {

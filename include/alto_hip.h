@@ -204,7 +204,10 @@ int alto_coo_mttkrp(int nmodes, const uint64_t* dims, uint64_t nnz, const uint64
  * groups of one rank, precision and tolerance, one iteration per group at a
  * time, members joining and leaving between iterations (0, default: every
  * job iterated by its worker); ALTO_TUNE_SERVE_LOCKSTEP_SESSIONS (default
- * 64): at most this many such jobs admitted at once.
+ * 64): at most this many such jobs admitted at once.  ALTO_TUNE_DENSE_F64:
+ * 1 (default) = FP32 / BF16 CP-ALS forms the Grams of its stored factors and
+ * the row solves U = M V^-1 on FP64 MFMA (the MTTKRP stays in the model's
+ * precision); 0 = on FP32 MFMA.
  * Streams are re-ordered on their next use. */
 enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, ALTO_TUNE_MTTKRP_VARIANT = 4,
        ALTO_TUNE_GRAPHS = 5, ALTO_TUNE_HOT_ROWS = 7, ALTO_TUNE_HOT_BLOCK = 8, ALTO_TUNE_HOT_LDS = 9,
@@ -212,7 +215,7 @@ enum { ALTO_TUNE_SEG = 1, ALTO_TUNE_CHUNK_LOG2 = 2, ALTO_TUNE_NONTEMPORAL = 3, A
        ALTO_TUNE_FIBERS = 13, ALTO_TUNE_FIBER_MIN = 14, ALTO_TUNE_FIBER_CHUNK_LOG2 = 15,
        ALTO_TUNE_FIBER_MIN_NNZ = 16, ALTO_TUNE_FIBER_MODE_RULE = 17, ALTO_TUNE_DETERMINISTIC = 18,
        ALTO_TUNE_APR_PI_CACHE = 19, ALTO_TUNE_LOCKSTEP_DENSE_BATCH = 20, ALTO_TUNE_LOCKSTEP_GRAPHS = 21,
-       ALTO_TUNE_SERVE_LOCKSTEP_NNZ = 22, ALTO_TUNE_SERVE_LOCKSTEP_SESSIONS = 23 };
+       ALTO_TUNE_SERVE_LOCKSTEP_NNZ = 22, ALTO_TUNE_SERVE_LOCKSTEP_SESSIONS = 23, ALTO_TUNE_DENSE_F64 = 24 };
 int alto_tune_set(int param, int value);
 
 /* Autotuner (no reference counterpart; the reference fixes its shapes at

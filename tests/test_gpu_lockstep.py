@@ -270,25 +270,35 @@ def test_lockstep_graph_with_atomic_reductions(prec, tol):
         A.check(lib.alto_tune_set(LOCK_GRAPHS, 0), "tune_set")
 
 
+@pytest.mark.parametrize("dense64", [1, 0])
 @pytest.mark.parametrize("prec,tol", [(1, 1e-5), (2, 1e-3)])
-def test_lockstep_reduced_precision_against_oracle(prec, tol):
-    """FP32 / BF16 sessions in lockstep against the oracle's FP64 CP-ALS from
-    the same initial model: the fit after 8 iterations within 1e-5 (FP32)
-    and 1e-3 (BF16)."""
+def test_reduced_precision_against_oracle(prec, tol, dense64):
+    """FP32 / BF16 sessions, alone and in lockstep, with the Grams and row
+    solves on FP64 (ALTO_TUNE_DENSE_F64, the default) or FP32 MFMA, against
+    the oracle's FP64 CP-ALS from the same initial model: the fit after 8
+    iterations within 1e-5 (FP32) and 1e-3 (BF16)."""
     import oracle
     A = _A()
-    specs = SPECS[:2] + SPECS[3:]
-    data, lock = [], []
-    for k, (dims, nnz, kind) in enumerate(specs):
-        gen = zipf_tensor if kind == "zipf" else uniform_tensor
-        cidx, vals = gen(dims, nnz, 400 + k)
-        g = A.AltoTensor(dims, cidx, vals, nprtn=8)
-        U, lam = oracle.random_init(dims, 16, 40 + k)
-        data.append((dims, cidx, vals, g))
-        lock.append(A.CpdSession(g, U, precision=prec))
-    assert A.cpd_sessions_iterate(lock, 8) == [8] * len(specs)
-    for k, ((dims, cidx, vals, _), sess) in enumerate(zip(data, lock)):
-        o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
-        Uo, lo = oracle.random_init(dims, 16, 40 + k)
-        _, fo = o.cpd_als(Uo, lo, 8, 0.0)
-        assert abs(sess.get()[2] - fo[-1]) <= tol, (dims, sess.get()[2], fo[-1])
+    A.check(A.lib().alto_tune_set(24, dense64), "tune_set")
+    try:
+        specs = SPECS[:2] + SPECS[3:]
+        data, alone, lock = [], [], []
+        for k, (dims, nnz, kind) in enumerate(specs):
+            gen = zipf_tensor if kind == "zipf" else uniform_tensor
+            cidx, vals = gen(dims, nnz, 400 + k)
+            g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+            U, lam = oracle.random_init(dims, 16, 40 + k)
+            data.append((dims, cidx, vals, g))
+            alone.append(A.CpdSession(g, U, precision=prec))
+            lock.append(A.CpdSession(g, U, precision=prec))
+        for x in alone:
+            x.iterate(8)
+        assert A.cpd_sessions_iterate(lock, 8) == [8] * len(specs)
+        for k, ((dims, cidx, vals, _), a, b) in enumerate(zip(data, alone, lock)):
+            o = oracle.OracleAlto(dims, cidx, vals, nprtn=8)
+            Uo, lo = oracle.random_init(dims, 16, 40 + k)
+            _, fo = o.cpd_als(Uo, lo, 8, 0.0)
+            assert abs(a.get()[2] - fo[-1]) <= tol, (dims, a.get()[2], fo[-1])
+            assert abs(b.get()[2] - fo[-1]) <= tol, (dims, b.get()[2], fo[-1])
+    finally:
+        A.check(A.lib().alto_tune_set(24, 1), "tune_set")
