@@ -50,10 +50,12 @@ def test_alone_equals_in_mix(prec, tol, lockstep_nnz):
     s8.close()
 
 
-def test_budget_admission_and_error_isolation():
+@pytest.mark.parametrize("lockstep_nnz", [0, 1 << 30])
+def test_budget_admission_and_error_isolation(lockstep_nnz):
     from alto_amd.serve import Server
     jobs = _jobs(6, seed=1)
-    s = Server(200 << 20, 4)  # tiny budget: jobs are admitted one at a time
+    # tiny budget: jobs are admitted one at a time
+    s = Server(200 << 20, 4, lockstep_nnz=lockstep_nnz)
     ids = [s.submit(j["dims"], list(j["cidx"]), j["vals"], j["rank"], 5, 0.0, j["seed"]) for j in jobs]
     # a job whose coordinates exceed its dims fails alone
     bad = jobs[0]
@@ -65,6 +67,12 @@ def test_budget_admission_and_error_isolation():
     assert s.result(bid)[0].status < 0
     st = s.stats()
     assert st.hbm_peak <= max(st.hbm_budget, max(s.result(i)[0].footprint for i in ids))
+    # finished and failed jobs can both be released; a released id is gone
+    from alto_amd import AltoError
+    assert s.L.alto_server_release(s.h, bid) == 0
+    assert s.L.alto_server_release(s.h, ids[0]) == 0
+    with pytest.raises(AltoError):
+        s.result(bid)
     s.close()
 
 
