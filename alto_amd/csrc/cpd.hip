@@ -87,8 +87,13 @@ template <typename UT, typename ST, typename MT>
 struct CpdState {
     int N = 0, R = 0, prec = 0;
     DevBuf U[ALTO_MAXN], S[ALTO_MAXN];
-    DevBuf M, scratch, grams, gpart, L, P, wsA, wsQ, status, psum, pmax, pin, lambda, ipart, fitbuf, nx, nxpart;
+    DevBuf M, scratch, grams, gpart, L, P, wsA, wsQ, status, psum, pmax, pin, pin_gram, lambda, ipart, fitbuf, nx,
+        nxpart;
     int nb_solve = 1;
+    // storage that rounds (BF16 factors in FP32 models): the fit's inner
+    // product is taken from the stored values in the last mode's Gram pass
+    // (pin_gram), so that the fit is that of the model the MTTKRPs see
+    static constexpr bool rounds = !std::is_same<UT, ST>::value;
     UT* u(int m) { return U[m].as<UT>(); }
     ST* st(int m) { return std::is_same<UT, ST>::value ? reinterpret_cast<ST*>(U[m].p) : S[m].as<ST>(); }
 };
@@ -117,6 +122,7 @@ static int cpd_alloc(alto_tensor* t, int R, CpdState<UT, ST, MT>& c)
     ALTO_TRY(c.psum.alloc((size_t)c.nb_solve * R * 8));
     ALTO_TRY(c.pmax.alloc((size_t)c.nb_solve * R * 8));
     ALTO_TRY(c.pin.alloc((size_t)c.nb_solve * 8));
+    ALTO_TRY(c.pin_gram.alloc((size_t)gram_blocks(maxd) * 8));
     ALTO_TRY(c.lambda.alloc((size_t)R * 8));
     ALTO_TRY(c.ipart.alloc((size_t)64 * R * 8));
     ALTO_TRY(c.fitbuf.alloc(4 * 8));
@@ -215,30 +221,41 @@ static int cpd_mode_rest(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it,
     ALTO_TRY(chol_device(c.grams.template as<double>(), N, j, R, c.L.template as<double>(), c.P.template as<double>(),
                          c.wsA.template as<double>(), c.wsQ.template as<double>(), c.status.template as<int>(), s));
     // the last mode's solve also leaves the fit's inner-product partials
+    // (storage that rounds: the Gram pass below does)
+    const bool last = j == N - 1;
     ALTO_TRY((solve_device<MT, UT>(c.M.template as<MT>(), t->dims[j], R, c.L.template as<double>(),
                                    c.P.template as<double>(), c.status.template as<int>(), c.u(j),
                                    c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, s,
-                                   j == N - 1 ? c.pin.template as<double>() : nullptr)));
+                                   last && !c.rounds ? c.pin.template as<double>() : nullptr)));
     ALTO_TRY(pt.mark(PH_PINV));
     ALTO_TRY(lambda_device(c.psum.template as<double>(), c.pmax.template as<double>(), c.nb_solve, R, it == 0 ? 0 : 1,
                            c.lambda.template as<double>(), s));
     ALTO_TRY(pt.mark(PH_NORM));
     // ... and clears M for the next mode's MTTKRP in the same launches
     const uint64_t next_rows = t->dims[(j + 1) % N];
-    ALTO_TRY((gram_scale_device<UT, ST>(c.u(j), t->dims[j], R, c.lambda.template as<double>(),
-                                        std::is_same<UT, ST>::value ? nullptr : c.st(j),
-                                        c.grams.template as<double>() + (size_t)j * R * R,
-                                        c.gpart.template as<double>(), s, gram_f64(c.prec), c.M.p,
-                                        next_rows * R * sizeof(MT))));
+    const bool inner = last && c.rounds;
+    static_assert(std::is_same<MT, double>::value, "the Gram pass reads an FP64 M");
+    ALTO_TRY((gram_scale_device<UT, ST>(
+        c.u(j), t->dims[j], R, c.lambda.template as<double>(), std::is_same<UT, ST>::value ? nullptr : c.st(j),
+        c.grams.template as<double>() + (size_t)j * R * R, c.gpart.template as<double>(), s, gram_f64(c.prec), c.M.p,
+        next_rows * R * sizeof(MT), inner ? c.M.template as<double>() : nullptr,
+        inner ? (R <= 64 ? c.pin_gram.template as<double>() : c.ipart.template as<double>()) : nullptr)));
     ALTO_TRY(pt.mark(PH_UPDATE));
     return ALTO_OK;
 }
 
-// fit after an iteration: the inner product came out of the last mode's solve
+// fit after an iteration: the inner product came out of the last mode's
+// solve (storage that rounds: out of its Gram pass)
 template <typename UT, typename ST, typename MT>
 static int cpd_fit_step(alto_tensor* t, CpdState<UT, ST, MT>& c, double normX, hipStream_t s)
 {
-    (void)t;
+    if (c.rounds && c.R > 64)
+        return fit_device(c.grams.template as<double>(), c.N, c.R, c.lambda.template as<double>(),
+                          c.ipart.template as<double>(), normX, c.fitbuf.template as<double>(), s);
+    if (c.rounds)
+        return fit_scalar_device(c.grams.template as<double>(), c.N, c.R, c.lambda.template as<double>(),
+                                 c.pin_gram.template as<double>(), gram_blocks(t->dims[c.N - 1]), normX,
+                                 c.fitbuf.template as<double>(), s);
     return fit_scalar_device(c.grams.template as<double>(), c.N, c.R, c.lambda.template as<double>(),
                              c.pin.template as<double>(), c.nb_solve, normX, c.fitbuf.template as<double>(), s);
 }
@@ -401,7 +418,10 @@ struct CpdSession : CpdSessionBase {
         x->U = c.u(j);
         x->psum = c.psum.template as<double>();
         x->pmax = c.pmax.template as<double>();
-        x->pin = j == N - 1 ? c.pin.template as<double>() : nullptr;
+        const bool last = j == N - 1;
+        x->pin = last && !c.rounds ? c.pin.template as<double>() : nullptr;
+        x->Mgram = last && c.rounds ? c.M.template as<double>() : nullptr;
+        x->pin_gram = last && c.rounds ? c.pin_gram.template as<double>() : nullptr;
         x->nb = c.nb_solve;
         x->which = it == 0 ? 0 : 1;
         x->lambda = c.lambda.template as<double>();
@@ -415,12 +435,12 @@ struct CpdSession : CpdSessionBase {
     {
         f->grams = c.grams.template as<double>();
         f->lambda = c.lambda.template as<double>();
-        f->pin = c.pin.template as<double>();
+        f->pin = c.rounds ? c.pin_gram.template as<double>() : c.pin.template as<double>();
         f->out = c.fitbuf.template as<double>();
         f->gather = gather;
         f->normX = normX;
         f->nmode = c.N;
-        f->nb = c.nb_solve;
+        f->nb = c.rounds ? gram_blocks(t->dims[c.N - 1]) : c.nb_solve;
     }
     int iterate(int max_iters, double eps, double* fit_trace, int* ran, int verbose) override
     {

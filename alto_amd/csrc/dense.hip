@@ -98,10 +98,15 @@ __global__ void __launch_bounds__(256) k_gram_partial(const T* __restrict__ U, u
 // place and writes the storage copy S (when the storage type differs), and
 // the Gram is taken of the stored values — the column normalisation and the
 // Gram update of a mode in one pass over U.
-template <typename T, typename ST, bool F64, int T16, bool SCALE>
+// INNER (with SCALE): the block also sums lambda_c S[i,c] Mi[i,c] over its
+// rows into pin[bid] -- the fit's inner product <X, model> taken from the
+// stored values (for storage that rounds, so that the fit is that of the
+// model the MTTKRPs and Grams see); skipped when Mi is null.
+template <typename T, typename ST, bool F64, int T16, bool SCALE, bool INNER>
 __device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
                                                double* __restrict__ part, const double* __restrict__ lambda,
-                                               ST* __restrict__ S, uint32_t bid)
+                                               ST* __restrict__ S, const double* __restrict__ Mi,
+                                               double* __restrict__ pin, uint32_t bid)
 {
     constexpr int NP = T16 * (T16 + 1) / 2;
     using acc_t = typename std::conditional<F64, d4_t, f4_t>::type;
@@ -118,6 +123,7 @@ __device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, in
 #pragma unroll
         for (int cb = 0; cb < T16; ++cb) lc[cb] = cb * 16 + r16 < R ? lambda[cb * 16 + r16] : 1.0;
     }
+    double inacc = 0;
     // two 16-row steps per pass: both steps' loads are issued before the products
     for (uint64_t i00 = base + 4 * wave; i00 < end; i00 += 32) {
         double raw[2][T16];
@@ -149,6 +155,8 @@ __device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, in
                             S[i * R + c] = sv;
                             x[cb] = as_f64(sv);
                         }
+                        if constexpr (INNER)
+                            if (Mi) inacc += x[cb] * lc[cb] * Mi[i * R + c];
                     }
                 } else {
                     x[cb] = raw[u][cb];
@@ -166,8 +174,16 @@ __device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, in
                 }
         }
     }
+    __shared__ double s_in[4];
+    if constexpr (INNER) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) inacc += __shfl_xor(inacc, o);
+        if (lane == 0) s_in[wave] = inacc;
+    }
     for (int k = threadIdx.x; k < R * R; k += blockDim.x) sG[k] = 0.0;
     __syncthreads();
+    if constexpr (INNER)
+        if (Mi && threadIdx.x == 0) pin[bid] = (s_in[0] + s_in[1]) + (s_in[2] + s_in[3]);
     for (int w = 0; w < 4; ++w) {
         if (wave == w) {
             int q = 0;
@@ -195,12 +211,13 @@ __device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, in
     }
 }
 
-template <typename T, typename ST, bool F64, int T16, bool SCALE>
+template <typename T, typename ST, bool F64, int T16, bool SCALE, bool INNER>
 __global__ void __launch_bounds__(256) k_gram_rows(T* __restrict__ U, uint64_t I, int R, uint64_t rows_per_block,
                                                    double* __restrict__ part, const double* __restrict__ lambda,
-                                                   ST* __restrict__ S)
+                                                   ST* __restrict__ S, const double* __restrict__ Mi,
+                                                   double* __restrict__ pin)
 {
-    gram_rows_body<T, ST, F64, T16, SCALE>(U, I, R, rows_per_block, part, lambda, S, blockIdx.x);
+    gram_rows_body<T, ST, F64, T16, SCALE, INNER>(U, I, R, rows_per_block, part, lambda, S, Mi, pin, blockIdx.x);
 }
 
 __global__ void k_sum_partials(const double* __restrict__ part, int nparts, int n, double* __restrict__ out)
@@ -292,11 +309,11 @@ int gram_device(const T* U, uint64_t I, int R, double* G, double* part, hipStrea
 #define GR_(TT)                                                                                                  \
     do {                                                                                                         \
         if (f64)                                                                                                 \
-            hipLaunchKernelGGL((k_gram_rows<T, T, true, TT, false>), dim3(nb), dim3(256), lds, s, Uw, I, R, rpb, \
-                               part, nullptr, nullptr);                                                          \
+            hipLaunchKernelGGL((k_gram_rows<T, T, true, TT, false, false>), dim3(nb), dim3(256), lds, s, Uw, I, R, \
+                               rpb, part, nullptr, nullptr, nullptr, nullptr);                                   \
         else                                                                                                     \
-            hipLaunchKernelGGL((k_gram_rows<T, T, false, TT, false>), dim3(nb), dim3(256), lds, s, Uw, I, R,     \
-                               rpb, part, nullptr, nullptr);                                                     \
+            hipLaunchKernelGGL((k_gram_rows<T, T, false, TT, false, false>), dim3(nb), dim3(256), lds, s, Uw, I, \
+                               R, rpb, part, nullptr, nullptr, nullptr, nullptr);                                \
     } while (0)
         switch ((R + 15) / 16) {
         case 1: GR_(1); break;
@@ -315,26 +332,32 @@ int gram_device(const T* U, uint64_t I, int R, double* G, double* part, hipStrea
 }
 template <typename UT, typename ST>
 int gram_scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, double* G, double* part, hipStream_t s,
-                      bool f64, void* zero, uint64_t zero_bytes)
+                      bool f64, void* zero, uint64_t zero_bytes, const double* Mi, double* pin)
 {
     constexpr bool same = std::is_same<UT, ST>::value;
     if (R > 64) {
         ALTO_TRY((scale_device<UT, ST>(U, I, R, lambda, same ? nullptr : S, s)));
         ALTO_TRY(gram_device<ST>(same ? reinterpret_cast<const ST*>(U) : S, I, R, G, part, s, f64));
+        if (Mi) ALTO_TRY((inner_partial_device<ST, double>(same ? reinterpret_cast<const ST*>(U) : S, Mi, I, R, pin, s)));
         if (zero_bytes) ALTO_TRY(sum_cols_device(nullptr, 0, 0, nullptr, zero, zero_bytes, s));
         return ALTO_OK;
     }
     const int nb = gram_blocks(I);
     const uint64_t rpb = ((I + nb - 1) / nb + 15) / 16 * 16;
     const size_t lds = (size_t)R * R * 8;
-#define GS_(TT)                                                                                                 \
-    do {                                                                                                        \
-        if (f64)                                                                                                \
-            hipLaunchKernelGGL((k_gram_rows<UT, ST, true, TT, true>), dim3(nb), dim3(256), lds, s, U, I, R, rpb, \
-                               part, lambda, S);                                                                \
-        else                                                                                                    \
-            hipLaunchKernelGGL((k_gram_rows<UT, ST, false, TT, true>), dim3(nb), dim3(256), lds, s, U, I, R,    \
-                               rpb, part, lambda, S);                                                           \
+#define GSI_(TT, F, IN)                                                                                        \
+    hipLaunchKernelGGL((k_gram_rows<UT, ST, F, TT, true, IN>), dim3(nb), dim3(256), lds, s, U, I, R, rpb, part, \
+                       lambda, S, Mi, pin)
+#define GS_(TT)                           \
+    do {                                  \
+        if (Mi) {                         \
+            if (f64) GSI_(TT, true, true); \
+            else GSI_(TT, false, true);    \
+        } else if (f64) {                 \
+            GSI_(TT, true, false);        \
+        } else {                          \
+            GSI_(TT, false, false);       \
+        }                                 \
     } while (0)
     switch ((R + 15) / 16) {
     case 1: GS_(1); break;
@@ -343,14 +366,15 @@ int gram_scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, dou
     default: GS_(4); break;
     }
 #undef GS_
+#undef GSI_
     return sum_cols_device(part, nb, R * R, G, zero, zero_bytes, s);
 }
 template int gram_scale_device<double, double>(double*, uint64_t, int, const double*, double*, double*, double*,
-                                               hipStream_t, bool, void*, uint64_t);
+                                               hipStream_t, bool, void*, uint64_t, const double*, double*);
 template int gram_scale_device<float, float>(float*, uint64_t, int, const double*, float*, double*, double*,
-                                             hipStream_t, bool, void*, uint64_t);
+                                             hipStream_t, bool, void*, uint64_t, const double*, double*);
 template int gram_scale_device<float, bf16_t>(float*, uint64_t, int, const double*, bf16_t*, double*, double*,
-                                              hipStream_t, bool, void*, uint64_t);
+                                              hipStream_t, bool, void*, uint64_t, const double*, double*);
 
 template int gram_device<double>(const double*, uint64_t, int, double*, double*, hipStream_t, bool);
 template int gram_device<float>(const float*, uint64_t, int, double*, double*, hipStream_t, bool);
@@ -819,7 +843,7 @@ __global__ void k_inner_partial(const UT* __restrict__ U, const MT* __restrict__
     if (t < RP * R) {
         const int c = t % R, ri = t / R;
         for (uint64_t i = (uint64_t)blockIdx.x * RP + ri; i < I; i += (uint64_t)gridDim.x * RP)
-            a += (double)U[i * R + c] * (double)Mm[i * R + c];
+            a += as_f64(U[i * R + c]) * (double)Mm[i * R + c];
     }
     s[t] = a;
     __syncthreads();
@@ -1018,6 +1042,8 @@ int inner_partial_device(const UT* U, const MT* Mm, uint64_t I, int R, double* p
 }
 template int inner_partial_device<double, double>(const double*, const double*, uint64_t, int, double*, hipStream_t);
 template int inner_partial_device<float, float>(const float*, const float*, uint64_t, int, double*, hipStream_t);
+template int inner_partial_device<bf16_t, double>(const bf16_t*, const double*, uint64_t, int, double*, hipStream_t);
+template int inner_partial_device<float, double>(const float*, const double*, uint64_t, int, double*, hipStream_t);
 
 int fit_device(const double* grams, int nmode, int R, const double* lambda, const double* ipart, double normX,
                double* out, hipStream_t s)
@@ -1088,6 +1114,8 @@ struct GramJob {
     void* S;
     const double* lambda;
     double* part;
+    const double* Mi;
+    double* pin;
     uint64_t I, rpb;
 };
 struct SumJob {
@@ -1127,13 +1155,13 @@ __global__ void __launch_bounds__(256) k_lambda_batch(BatchTab<LambdaJob> t, int
     lambda_body(j.psum, j.pmax, j.nb, R, j.which, j.lambda, blockIdx.x % R);
 }
 
-template <typename T, typename ST, bool F64, int T16>
+template <typename T, typename ST, bool F64, int T16, bool INNER>
 __global__ void __launch_bounds__(256) k_gram_rows_batch(BatchTab<GramJob> t, int R)
 {
     const int k = batch_job(t, blockIdx.x);
     const GramJob& j = t.a[k];
-    gram_rows_body<T, ST, F64, T16, true>((T*)j.U, j.I, R, j.rpb, j.part, j.lambda, (ST*)j.S,
-                                          blockIdx.x - t.start[k]);
+    gram_rows_body<T, ST, F64, T16, true, INNER>((T*)j.U, j.I, R, j.rpb, j.part, j.lambda, (ST*)j.S, j.Mi, j.pin,
+                                                 blockIdx.x - t.start[k]);
 }
 
 __global__ void __launch_bounds__(256) k_sum_cols_batch(BatchTab<SumJob> t, int n)
@@ -1174,7 +1202,7 @@ int dense_mode_batch(int n, const DenseModeJob* m, int R, bool f64, hipStream_t 
         la.a[k] = LambdaJob{x.psum, x.pmax, x.lambda, x.nb, x.which};
         const int gb = gram_blocks(x.I);
         const uint64_t rpb = ((x.I + gb - 1) / gb + 15) / 16 * 16;
-        gr.a[k] = GramJob{x.U, same ? nullptr : x.S, x.lambda, x.gpart, x.I, rpb};
+        gr.a[k] = GramJob{x.U, same ? nullptr : x.S, x.lambda, x.gpart, x.Mgram, x.pin_gram, x.I, rpb};
         gr.start[k + 1] = gr.start[k] + (uint32_t)gb;
         // the sums and the clearing of the next mode's M, as sum_cols_device
         const int nsum = (R * R + 31) / 32;
@@ -1208,13 +1236,16 @@ int dense_mode_batch(int n, const DenseModeJob* m, int R, bool f64, hipStream_t 
 #undef AP_
     hipLaunchKernelGGL(k_lambda_batch, dim3(n * R), dim3(256), 0, s, la, R);
     const size_t lds_gr = (size_t)R * R * 8;
+    // storage that rounds (BF16): the last modes' fit inner products from
+    // the stored values, in the same launch
 #define GS_(TT)                                                                                                     \
     do {                                                                                                            \
         if (f64)                                                                                                    \
-            hipLaunchKernelGGL((k_gram_rows_batch<UT, ST, true, TT>), dim3(gr.start[n]), dim3(256), lds_gr, s, gr, R); \
+            hipLaunchKernelGGL((k_gram_rows_batch<UT, ST, true, TT, !same>), dim3(gr.start[n]), dim3(256), lds_gr, s, \
+                               gr, R);                                                                              \
         else                                                                                                        \
-            hipLaunchKernelGGL((k_gram_rows_batch<UT, ST, false, TT>), dim3(gr.start[n]), dim3(256), lds_gr, s, gr, \
-                               R);                                                                                  \
+            hipLaunchKernelGGL((k_gram_rows_batch<UT, ST, false, TT, !same>), dim3(gr.start[n]), dim3(256), lds_gr, \
+                               s, gr, R);                                                                           \
     } while (0)
     switch ((R + 15) / 16) {
     case 1: GS_(1); break;

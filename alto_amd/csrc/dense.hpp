@@ -20,10 +20,14 @@ template <typename UT, typename ST>
 int scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, hipStream_t s);
 // U /= lambda (per column) with the storage copy S (null when the types
 // agree), then the Gram of the stored values, in one pass for R <= 64;
-// zero_bytes of `zero` are cleared in the same launches
+// zero_bytes of `zero` are cleared in the same launches.  With Mi (this
+// mode's MTTKRP), also the fit's inner product from the stored values:
+// R <= 64: per block sum of lambda_c S[i,c] Mi[i,c] into pin[gram_blocks(I)];
+// R > 64: per-column partials (64 x R, unweighted) into pin
 template <typename UT, typename ST>
 int gram_scale_device(UT* U, uint64_t I, int R, const double* lambda, ST* S, double* G, double* part, hipStream_t s,
-                      bool f64, void* zero = nullptr, uint64_t zero_bytes = 0);
+                      bool f64, void* zero = nullptr, uint64_t zero_bytes = 0, const double* Mi = nullptr,
+                      double* pin = nullptr);
 // column sums of partials and the clearing of a buffer, one launch
 int sum_cols_device(const double* part, int nparts, int n, double* out, void* zero, uint64_t zero_bytes,
                     hipStream_t s);
@@ -60,6 +64,10 @@ struct DenseModeJob {
     double *G, *gpart;
     void* zero;
     uint64_t zero_bytes;
+    // storage that rounds: this mode's MTTKRP and the per-block fit inner
+    // products taken from the stored values (null: none)
+    const double* Mgram;
+    double* pin_gram;
 };
 constexpr int DENSE_BATCH_MAX = 16;
 // the mode updates of n <= DENSE_BATCH_MAX models of one rank (<= 64) and

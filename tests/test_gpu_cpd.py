@@ -202,3 +202,88 @@ def test_cpd_fit_of_returned_model_equals_last_fit():
     Uo, lo = oracle.random_init(dims, R, 3)
     no, fo = o.cpd_als(Uo, lo, 6, 0.0)
     np.testing.assert_allclose(f.value, fo[-1], rtol=1e-9, atol=1e-12)
+
+
+def _bf16_round(x):
+    """FP32 -> BF16 with round to nearest even, back to float64."""
+    a = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((a + 0x7FFF + ((a >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _stored_model_fit(A, g, R, U, lam):
+    Ub = [np.ascontiguousarray(_bf16_round(u)) for u in U]
+    f = ctypes.c_double(0)
+    A.check(A.lib().alto_cpd_fit(g.h, R, A._ptr_array([u.ctypes.data for u in Ub]), lam.ctypes.data, ctypes.byref(f)),
+            "cpd_fit")
+    return f.value
+
+
+@pytest.mark.parametrize("lockstep", [False, True])
+@pytest.mark.parametrize("R", [8, 33, 64, 80])
+def test_bf16_fit_is_the_fit_of_the_stored_model(R, lockstep):
+    """A BF16 model's fit (the inner product taken from the stored BF16
+    factors in the last mode's Gram pass; rank 80 through the separate
+    inner-product kernel) equals the FP64 fit of the model it returns with
+    its factors rounded to BF16 -- the model its MTTKRPs use -- within the
+    FP32 accumulation of its MTTKRP (1e-5 relative)."""
+    A = _A()
+    rng = np.random.default_rng(R)
+    cases = [([120, 90, 300], 30000, "zipf"), ([40, 30, 20, 25], 9000, "uniform"), ([2000, 50, 900], 20000, "zipf")]
+    sessions, tensors = [], []
+    for k, (dims, nnz, kind) in enumerate(cases):
+        gen = zipf_tensor if kind == "zipf" else uniform_tensor
+        cidx, vals = gen(dims, nnz, 70 + k)
+        vals = rng.lognormal(size=vals.shape[0])
+        g = A.AltoTensor(dims, cidx, vals, nprtn=8)
+        U, _ = oracle.random_init(dims, R, 5 + k)
+        tensors.append((dims, g))
+        sessions.append(A.CpdSession(g, U, precision=2))
+    if lockstep and R <= 64:
+        A.cpd_sessions_iterate(sessions, 10)
+    else:
+        for x in sessions:
+            x.iterate(10)
+    for (dims, g), x in zip(tensors, sessions):
+        U, lam, fit, _ = x.get()
+        f = _stored_model_fit(A, g, R, U, lam)
+        assert abs(f - fit) <= 1e-5 * max(1.0, abs(fit)), (dims, f, fit)
+
+
+@pytest.mark.parametrize("job", [1, 15, 31])
+def test_bf16_fit_on_serving_jobs_with_negative_residual(job):
+    """Jobs of the seeded serving stream (duplicate coordinates kept,
+    lognormal values) whose residual goes negative, so that the fit formula
+    gives 55-2800 and amplifies any inconsistency: the BF16 fit equals the
+    stored model's within 1e-5 relative (a fit whose inner product came
+    from the unrounded factors was 1.4e-3 - 2.2e-3 off on these jobs) and
+    the FP64 fit within 1e-3 relative."""
+    import torch
+    from alto_amd.serve import _generate, job_stream
+    A = _A()
+    j = dict(job_stream(job + 1, seed=5)[job])
+    coords, vals, nnz = _generate(j)
+    torch.cuda.synchronize()
+    g = A.AltoTensor.from_device(j["dims"], [c.data_ptr() for c in coords], vals.data_ptr(), nnz, nprtn=1)
+    R = j["rank"]
+    fits = {}
+    for prec in (0, 2):
+        h = ctypes.c_void_p()
+        A.check(A.lib().alto_cpd_session_create_seeded(g.h, R, prec, j["seed"] % (2 ** 31), None, ctypes.byref(h)),
+                "session")
+        tr = np.zeros(20)
+        ran, conv = ctypes.c_int(0), ctypes.c_int(0)
+        A.check(A.lib().alto_cpd_session_iterate(h, 20, 1e-5, tr.ctypes.data, ctypes.byref(ran), ctypes.byref(conv),
+                                                 0), "iterate")
+        U = [np.zeros((d, R)) for d in j["dims"]]
+        lam = np.zeros(R)
+        f = ctypes.c_double(0)
+        A.check(A.lib().alto_cpd_session_get(h, A._ptr_array([u.ctypes.data for u in U]), lam.ctypes.data,
+                                             ctypes.byref(f), None), "get")
+        A.lib().alto_cpd_session_destroy(h)
+        fits[prec] = (tr[ran.value - 1], U, lam)
+    fb, Ub, lb = fits[2]
+    assert abs(fb) > 10  # the regime where the residual is negative
+    f_exact = _stored_model_fit(A, g, R, Ub, lb)
+    assert abs(f_exact - fb) <= 1e-5 * abs(fb), (f_exact, fb)
+    assert abs(fb - fits[0][0]) <= 1e-3 * abs(fits[0][0]), (fb, fits[0][0])
