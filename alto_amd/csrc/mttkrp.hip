@@ -243,10 +243,11 @@ __global__ void __launch_bounds__(256)
     const uint64_t e = s + a.seg < a.nnz ? s + a.seg : a.nnz;
     const int R = a.rank;
 
-    double acc[E];
+    double acc[E], b_last[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) acc[q] = 0;
+    for (int q = 0; q < E; ++q) acc[q] = b_last[q] = 0;
     uint32_t cur = 0xffffffffu;
+    uint32_t b_row = 0xffffffffu;  // the model row b_last holds (this group's)
     // this lane's nonzero of the next batch (past the segment: row 0 of
     // every mode, value 0 -- valid memory, never accumulated)
     uint64_t kn[PW];
@@ -274,10 +275,26 @@ __global__ void __launch_bounds__(256)
                 constexpr int jj = decltype(ji)::value;
                 rj[jj] = bcast32<G, j0 + jj>(rown);
                 v[jj] = bcast_val<G, j0 + jj>(myv);
-                load_cols<double, double, G, E>(B + (uint64_t)rj[jj] * R, gl, R, b[jj]);
 #pragma unroll
                 for (int q = 0; q < E; ++q) pi[jj][q] = 1.0;
             });
+            // the model row only at the positions where some group's row
+            // changes: a wave-uniform branch, so every lane loads (groups
+            // whose row did not change reload the row they hold), and the
+            // stream's runs of one row skip the load
+            double rb[J][E];
+            uint32_t ldm = 0;  // positions loaded (uniform)
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const bool head = jj == 0 ? rj[0] != b_row : rj[jj] != rj[jj - 1];
+                if (__any(head)) {
+                    load_cols<double, double, G, E>(B + (uint64_t)rj[jj] * R, gl, R, rb[jj]);
+                    ldm |= 1u << jj;
+                }
+            }
+            // the mask reaches the selection below through an opaque copy, so
+            // that the selection is not folded into the branches above
+            asm volatile("" : "+s"(ldm));
             if constexpr (PIM == 2) {
 #pragma unroll
                 for (int jj = 0; jj < J; ++jj) {
@@ -323,6 +340,17 @@ __global__ void __launch_bounds__(256)
                     vn = a.val[i];
                 }
             }
+            // each position's model row is that of the last load at or before it
+#pragma unroll
+            for (int jj = 0; jj < J; ++jj) {
+                const bool use = (ldm >> jj) & 1u;
+#pragma unroll
+                for (int q = 0; q < E; ++q) {
+                    b_last[q] = use ? rb[jj][q] : b_last[q];
+                    b[jj][q] = b_last[q];
+                }
+            }
+            b_row = rj[J - 1];
 #pragma unroll
             for (int jj = 0; jj < J; ++jj) {
                 double d = 0;
