@@ -57,6 +57,7 @@ struct Job {
     int want_factors = 0;
     uint64_t footprint = 0;
     bool lockstep = false;  // iterated by the stepper
+    bool stepping = false;  // handed to the stepper (counted as running)
     // results
     int status = 0;  // 0 queued, 1 running, 2 done, < 0 error code
     int iters = 0;
@@ -109,6 +110,7 @@ struct alto_server {
     };
     uint64_t lock_max_nnz = 0;  // 0: off
     int lock_max = 0, lock_active = 0;
+    int stepping = 0;  // jobs handed to the stepper, not yet finished
     std::vector<Member> inbox;  // sessions created, waiting to join a group
     std::thread stepper;
     hipStream_t step_stream = nullptr;  // the groups' iterations
@@ -198,6 +200,7 @@ struct alto_server {
             j->status = status;
             in_use -= j->footprint;
             if (j->lockstep) --lock_active;
+            if (j->stepping) --stepping;
         }
         cv_work.notify_all();
         cv_done.notify_all();
@@ -227,6 +230,8 @@ struct alto_server {
                     {
                         std::lock_guard<std::mutex> g(mu);
                         inbox.push_back(Member{j, t, sess});
+                        j->stepping = true;
+                        ++stepping;
                         --running;
                     }
                     cv_step.notify_all();
@@ -408,7 +413,7 @@ int alto_server_stats(alto_server_t* S, alto_server_stats_t* st)
     if (!S || !st) return ALTO_ERR_ARG;
     std::lock_guard<std::mutex> g(S->mu);
     st->queued = (int)S->queue.size();
-    st->running = S->running;
+    st->running = S->running + S->stepping;  // with the jobs in lockstep groups
     st->hbm_in_use = S->in_use;
     st->hbm_peak = S->peak;
     st->hbm_budget = S->budget;
