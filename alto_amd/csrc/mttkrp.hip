@@ -384,10 +384,12 @@ __global__ void __launch_bounds__(256)
 // row is loaded only where the row changes inside a block of J nonzeros or
 // from the previous block, with the decision uniform over the wave (the
 // stream is ordered by row inside each chunk, so long runs skip most
-// model-row loads).
-template <int G, int PW, int NO>
+// model-row loads).  PIM as in k_phi_b: 0 gather the other rows, 1 gather
+// and store their products per nonzero into Pc, 2 read the products back
+// (one 16-byte load per lane instead of NO).
+template <int G, int PW, int NO, int PIM>
 __global__ void __launch_bounds__(256)
-    k_phi16(StreamArgs<double, double, double> a, const double* __restrict__ Bm, double eps)
+    k_phi16(StreamArgs<double, double, double> a, const double* __restrict__ Bm, double eps, double* __restrict__ Pc)
 {
     constexpr int E = 2;
     constexpr int NS = G < 4 ? 4 / G : 1;
@@ -450,19 +452,34 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                 for (int q = 0; q < E; ++q) pi[jj][q] = 1.0;
             });
-            static_for<0, NO>([&](auto oi) {
-                constexpr int o = decltype(oi)::value;
-                const double* __restrict__ U = a.Uo[o];
-                static_for<0, J>([&](auto ji) {
-                    constexpr int jj = decltype(ji)::value;
-                    constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
-                    const uint32_t off = bcast32<G, src>(offs[slot][o]);
-                    double x[E];
-                    VecLoad<double, E>::load(U + off + c0, x);
+            if constexpr (PIM == 2) {
+                // lanes past the rank read the row's first pair (never used)
 #pragma unroll
-                    for (int q = 0; q < E; ++q) pi[jj][q] *= x[q];
+                for (int jj = 0; jj < J; ++jj) {
+                    const uint64_t i = p + j0 + jj < e ? p + j0 + jj : e - 1;
+                    VecLoad<double, E>::load(Pc + i * R + c0, pi[jj]);
+                }
+            } else {
+                static_for<0, NO>([&](auto oi) {
+                    constexpr int o = decltype(oi)::value;
+                    const double* __restrict__ U = a.Uo[o];
+                    static_for<0, J>([&](auto ji) {
+                        constexpr int jj = decltype(ji)::value;
+                        constexpr int slot = (j0 + jj) / G, src = (j0 + jj) % G;
+                        const uint32_t off = bcast32<G, src>(offs[slot][o]);
+                        double x[E];
+                        VecLoad<double, E>::load(U + off + c0, x);
+#pragma unroll
+                        for (int q = 0; q < E; ++q) pi[jj][q] *= x[q];
+                    });
                 });
-            });
+            }
+            if constexpr (PIM == 1) {
+#pragma unroll
+                for (int jj = 0; jj < J; ++jj)
+                    if (inr && j0 + jj < cnt)
+                        *reinterpret_cast<v2f64*>(Pc + (p + j0 + jj) * R + c0) = v2f64{pi[jj][0], pi[jj][1]};
+            }
             // model rows where the row changes (uniform over the wave)
             double br[J][E];
             uint32_t ldm = 0;
@@ -653,9 +670,15 @@ int phi_device_cached(alto_tensor* t, int mode, int R, const double* const* U, d
     dim3 grid = stream_grid(t->nnz, seg, group_lanes(R), a);
     bool al16 = R % 2 == 0 && ((uintptr_t)U[mode] & 15) == 0;
     for (int o = 0; o < a.nother; ++o) al16 = al16 && ((uintptr_t)a.Uo[o] & 15) == 0;
-    if (g_phi_variant == 2 && al16 && !pim) {
+    if (g_phi_variant == 2 && al16 && (!pim || ((uintptr_t)pc & 15) == 0)) {
         const int no = t->nmode - 1;
-#define PHI16_NO_(N_) hipLaunchKernelGGL((k_phi16<G, PW, N_>), grid, dim3(256), 0, s, a, U[mode], eps)
+#define PHI16_P_(N_, P_) hipLaunchKernelGGL((k_phi16<G, PW, N_, P_>), grid, dim3(256), 0, s, a, U[mode], eps, pc)
+#define PHI16_NO_(N_)                      \
+    do {                                   \
+        if (pim == 1) PHI16_P_(N_, 1);     \
+        else if (pim == 2) PHI16_P_(N_, 2); \
+        else PHI16_P_(N_, 0);              \
+    } while (0)
         RANK_DISPATCH(R, t->pack.pw, {
             if (no == 1) PHI16_NO_(1);
             else if (no == 2) PHI16_NO_(2);
@@ -663,6 +686,7 @@ int phi_device_cached(alto_tensor* t, int mode, int R, const double* const* U, d
             else PHI16_NO_(4);
         });
 #undef PHI16_NO_
+#undef PHI16_P_
     } else if (g_phi_variant >= 1 || pim) {
         const int no = t->nmode - 1;
 #define PHI_NO_(N_)                                                                                              \

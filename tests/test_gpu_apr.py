@@ -66,3 +66,30 @@ def test_cp_apr_pi_cache_equals_gathers():
         assert abs(r1["loglik"] - r0["loglik"]) <= 1e-10 * abs(r0["loglik"])
         for a, b in zip(U1, U0):
             np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("R", [6, 7, 10, 16])
+def test_cp_apr_phi16_with_pi_cache_matches_oracle(R):
+    """The 16-byte-lane Phi (ALTO_TUNE_PHI_VARIANT = 2) with the product
+    cache (store, then stream) against the oracle's CP-APR, 3- and 4-mode
+    tensors, 64- and 128-bit keys; odd ranks fall back to the 8-byte-lane
+    kernel."""
+    import alto_amd as A
+    A.check(A.lib().alto_tune_set(11, 2), "tune_set")
+    try:
+        for dims, nnz, W, bench in (([40, 300, 500], 40000, 1, True), ([30, 20, 10, 40], 15000, 2, False)):
+            cidx, vals = _counts(dims, nnz, 31 + R)
+            g = A.AltoTensor(dims, cidx, vals, nprtn=8, lit_bits=64 * W, val_kind=A.VAL_U64)
+            o = oracle.OracleAlto(dims, cidx, vals, W=W, nprtn=8, val_int=True)
+            U, lam = oracle.random_init(dims, R, 4)
+            Uo, lo = oracle.random_init(dims, R, 4)
+            eps = 0.0 if bench else 1e-4
+            r = g.cp_apr(U, lam, 4, 5, epsilon=eps, eps_div_zero=0.0 if bench else 1e-10, dobench=bench)
+            ro = o.cp_apr(Uo, lo, 4, 5, eps=eps, eps_div_zero=0.0 if bench else 1e-10, dobench=bench)
+            assert r["outer_iters"] == ro["outer_iters"] and r["total_inner"] == ro["total_inner"]
+            np.testing.assert_allclose(r["loglik"], ro["loglik"], rtol=1e-9)
+            np.testing.assert_allclose(lam, lo, rtol=1e-9)
+            for a, b in zip(U, Uo):
+                np.testing.assert_allclose(a, b, rtol=1e-8, atol=1e-12)
+    finally:
+        A.check(A.lib().alto_tune_set(11, 1), "tune_set")
