@@ -118,10 +118,16 @@ __device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, in
     for (int q = 0; q < NP; ++q) acc[q] = acc_t{0, 0, 0, 0};
     const uint64_t base = (uint64_t)bid * rows_per_block;
     const uint64_t end = base + rows_per_block < I ? base + rows_per_block : I;
-    double lc[T16];
+    // FP32 / BF16 factors are scaled by the reciprocal of lambda (within
+    // their rounding); FP64 factors are divided, as the reference divides
+    constexpr bool RECIP = !std::is_same<T, double>::value;
+    double lc[T16], lr[T16];
     if constexpr (SCALE) {
 #pragma unroll
-        for (int cb = 0; cb < T16; ++cb) lc[cb] = cb * 16 + r16 < R ? lambda[cb * 16 + r16] : 1.0;
+        for (int cb = 0; cb < T16; ++cb) {
+            lc[cb] = cb * 16 + r16 < R ? lambda[cb * 16 + r16] : 1.0;
+            if constexpr (RECIP) lr[cb] = 1.0 / lc[cb];
+        }
     }
     double inacc = 0;
     // two 16-row steps per pass: both steps' loads are issued before the products
@@ -146,7 +152,7 @@ __device__ __forceinline__ void gram_rows_body(T* __restrict__ U, uint64_t I, in
                 if constexpr (SCALE) {
                     x[cb] = 0.0;
                     if (i < end && c < R) {
-                        const double v = raw[u][cb] / lc[cb];
+                        const double v = RECIP ? raw[u][cb] * lr[cb] : raw[u][cb] / lc[cb];
                         U[i * R + c] = (T)v;
                         if constexpr (std::is_same<T, ST>::value) {
                             x[cb] = (double)(T)v;
