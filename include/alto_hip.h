@@ -275,15 +275,18 @@ int alto_cpd_session_iterate(alto_cpd_session_t* s, int max_iters, double epsilo
 /* Several sessions iterated in lockstep on one stream (no reference
  * counterpart: the reference runs one cpd_alto per process).  Each
  * iteration runs, for every mode index j, one alto_mttkrp_batched over the
- * sessions that have a mode j and have not converged, then each of their
- * mode-j solves (spread over 4 internal streams, joined back to `stream`
- * before the next MTTKRP), then all their fits, read with one
- * synchronization.  Every
+ * sessions that have a mode j and have not converged, then their mode-j
+ * updates -- at rank <= 64 each kernel of the update once for all of them
+ * (R x R inverse, row solves, lambda, Gram + scaling; groups of 16), above
+ * that one chain per session over 4 internal streams joined back to
+ * `stream` -- then all their fits, read with one synchronization
+ * (ALTO_TUNE_LOCKSTEP_DENSE_BATCH, ALTO_TUNE_LOCKSTEP_GRAPHS).  Every
  * session ends where alto_cpd_session_iterate(max_iters, epsilon) would leave
- * it, up to the order of the MTTKRP's floating-point atomic additions.  All
- * sessions share the rank and the precision; earlier work on each session's
- * own stream is waited for; the sessions' graphs and phase timers are not
- * used.  iters (may be null): iterations run per session. */
+ * it, up to the order of the MTTKRP's floating-point atomic additions (bit
+ * for bit with ALTO_TUNE_DETERMINISTIC).  All sessions share the rank and
+ * the precision; earlier work on each session's own stream is waited for;
+ * the sessions' graphs and phase timers are not used.  iters (may be null):
+ * iterations run per session. */
 int alto_cpd_sessions_iterate(int n, alto_cpd_session_t* const* sessions, int max_iters, double epsilon,
                               alto_stream_t stream, int* iters);
 int alto_cpd_session_get(alto_cpd_session_t* s, double* const* U, double* lambda, double* fit,
