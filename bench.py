@@ -426,6 +426,23 @@ def c1_leg(with_cpu):
     torch.cuda.synchronize()
     als_s = time.perf_counter() - t
     del sess
+    # the same 20 iterations, each replayed as a HIP graph (ALTO_TUNE_GRAPHS;
+    # the first and the later iterations are captured once each, outside the
+    # timed region, on a warm-up session)
+    A.check(A.lib().alto_tune_set(5, 1), "tune_set")
+    try:
+        sess = A.CpdSession(T, U0, precision=A.PREC_F64, stream=stream)
+        sess.iterate(2)
+        del sess
+        sess = A.CpdSession(T, U0, precision=A.PREC_F64, stream=stream)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        gfits, _ = sess.iterate(20, epsilon=-1.0)
+        torch.cuda.synchronize()
+        graph_s = time.perf_counter() - t
+        del sess
+    finally:
+        A.check(A.lib().alto_tune_set(5, 0), "tune_set")
     Uf = [torch.from_numpy(u).cuda() for u in U0]
     out = torch.zeros(dims[0], R, dtype=torch.float64, device="cuda")
     uptr = A._ptr_array([u.data_ptr() for u in Uf])
@@ -439,8 +456,11 @@ def c1_leg(with_cpu):
     mtt_s = e0.elapsed_time(e1) * 1e-3
     res = {"dims": dims, "nnz": nnz, "rank": R,
            "gpu": {"cpals_20_iters_s": als_s, "iters_per_s": 20 / als_s, "final_fit": float(fits[-1]),
-                   "mttkrp_mode0_100_s": mtt_s, "mttkrp_mode0_ms": mtt_s * 10}}
-    log(f"c1: GPU 20 CP-ALS iterations {als_s * 1e3:.1f} ms, 100 mode-0 MTTKRPs {mtt_s * 1e3:.1f} ms")
+                   "mttkrp_mode0_100_s": mtt_s, "mttkrp_mode0_ms": mtt_s * 10,
+                   "graph_replay": {"cpals_20_iters_s": graph_s, "iters_per_s": 20 / graph_s,
+                                    "final_fit_abs_diff_vs_eager": abs(float(gfits[-1]) - float(fits[-1]))}}}
+    log(f"c1: GPU 20 CP-ALS iterations {als_s * 1e3:.1f} ms (replayed as graphs {graph_s * 1e3:.1f} ms), "
+        f"100 mode-0 MTTKRPs {mtt_s * 1e3:.1f} ms")
     if with_cpu:
         import oracle
         threads, cores, model = host_cpu()
