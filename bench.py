@@ -177,6 +177,8 @@ def main():
                     help="profiling driver: only the 16-job batched MTTKRP comparison, its JSON on stdout")
     ap.add_argument("--apr-only", action="store_true",
                     help="profiling driver: only the CP-APR workload, its JSON on stdout")
+    ap.add_argument("--c1-only", action="store_true",
+                    help="profiling driver: only the 1000^3 1M-nonzero rank-16 GPU runs, their JSON on stdout")
     ap.add_argument("--serve-jobs", type=int, default=2000,
                     help="jobs of the Poisson(50/s) serving stream to run (0 = skip; the full stream is 2000)")
     ap.add_argument("--tune", action="append", default=[], metavar="PARAM=VALUE",
@@ -214,6 +216,9 @@ def main():
         return
     if args.apr_only:
         print(json.dumps(apr_leg()), flush=True)
+        return
+    if args.c1_only:
+        print(json.dumps(c1_leg(False)), flush=True)
         return
 
     t0 = time.time()
