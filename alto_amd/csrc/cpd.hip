@@ -44,8 +44,11 @@ struct PhaseTimer {
     {
         if (!on) return ALTO_OK;
         if (used == ev.size()) {
+            // timing only: the marks are read after the stream has been
+            // synchronized, so no mark needs the system-scope release (an L2
+            // write-back per mark, which stalls the kernels behind it)
             hipEvent_t e = nullptr;
-            HIP_TRY(hipEventCreate(&e));
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             ev.push_back(e);
             phase.push_back(-1);
         }
