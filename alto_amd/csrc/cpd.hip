@@ -208,7 +208,9 @@ static int cpd_mode_step(alto_tensor* t, CpdState<UT, ST, MT>& c, int j, int it,
     const int R = c.R, N = c.N;
     const void* fptr[ALTO_MAXN];
     for (int m = 0; m < N; ++m) fptr[m] = c.st(m);
-    ALTO_TRY(pt.mark(-1));
+    // the modes of an iteration run back to back: mode j > 0 starts at the
+    // previous mode's last mark
+    if (j == 0) ALTO_TRY(pt.mark(-1));
     // M is zero here: cleared at set-up and by the previous mode's last launch
     ALTO_TRY(mttkrp_device(t, j, R, fptr, c.M.p, c.prec, 1, s, 0));
     ALTO_TRY(pt.mark(PH_MTTKRP));
@@ -455,7 +457,7 @@ struct CpdSession : CpdSessionBase {
             const double ti = now_s();
             const double mt_before = acc[PH_MTTKRP];
             for (int j = 0; j < c.N; ++j) ALTO_TRY(cpd_mode_step(t, c, j, it, pt, s));
-            ALTO_TRY(pt.mark(-1));
+            // the fit starts at the last mode's last mark
             ALTO_TRY(cpd_fit_step(t, c, normX, s));
             ALTO_TRY(pt.mark(PH_FIT));
             double fb[3];
